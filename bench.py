@@ -1,0 +1,142 @@
+"""Photon-transport throughput on one MI355X.
+
+A step is one persistent launch of the transport kernel over N packets of
+the slab-between-glass-slides workload (mu_a=1/cm, mu_s=100/cm, g=0.9,
+n=1.40, 0.1 cm sample between 0.1 cm n=1.52 slides, dual spheres of radius
+4 cm with 0.5 cm sample/source ports, 0.25 cm detector port, wall 0.98;
+FP32 transport, Philox seed 1234 + step).  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--packets 1073741824]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# VALU lane-slots per event of a packet history (simple op = 1, quarter-rate
+# transcendental = 4, one Philox4x32-10 uniform = 37):
+#   interaction 174, interface hit 87, entrance 100, sphere decision 40,
+#   re-injection 130, packet set-up 10.
+SLOTS = {"interactions": 174, "boundary_hits": 87, "entrances": 100,
+         "sphere_decisions": 40, "reinjections": 130}
+SLOTS_PER_PACKET = 10
+
+
+def slots_per_packet(events, n):
+    return SLOTS_PER_PACKET + sum(SLOTS[k] * events[k] for k in SLOTS) / float(n)
+
+
+def cpu_baseline(seconds):
+    """Time the pure-Python restatement of the reference CPU propagator
+    (single-threaded, like ISMLMC.py) on a bounded sample of the same
+    workload, and the C oracle on all host cores."""
+    from ismc_amd.workloads import slab_between_slides
+    from oracle import ismc_py, oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import spec_of
+    spec = spec_of(slab_between_slides())
+    n, t0 = 0, time.perf_counter()
+    chunk = 2000
+    while time.perf_counter() - t0 < seconds:
+        ismc_py.simulate(spec, chunk, seed=n)
+        n += chunk
+    py_rate = n / (time.perf_counter() - t0)
+    cores = len(os.sched_getaffinity(0))
+    threads = min(cores, 16)
+    m = 2_000_000
+    t0 = time.perf_counter()
+    oracle.simulate(spec, m, seed=1, threads=threads)
+    c_rate = m / (time.perf_counter() - t0)
+    return {"value": py_rate, "unit": "packets/s", "cores": 1, "kind": "port",
+            "sample": "%d packets of the pure-Python restatement (ISMLMC.py/MLMC.py structure, "
+                      "math+random) of the same slab-between-slides workload" % n,
+            "c_oracle": {"value": c_rate, "unit": "packets/s", "cores": threads,
+                         "sample": "%d packets, C oracle FP64, OpenMP" % m}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--packets", type=int, default=1 << 30)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X "
+              "(packets shard by packet id across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from ismc_amd import Engine
+    from ismc_amd.workloads import slab_between_slides, three_layer_sample
+
+    eng = Engine(0)
+    info = eng.device_info()
+    cfg = slab_between_slides()
+    N = int(a.packets)
+    print("bench: %s (%s, %d CUs @ %d MHz), %d packets/step" %
+          (info["name"], info["arch"], info["n_cu"], info["clock_khz"] // 1000, N),
+          file=sys.stderr, flush=True)
+    for i in range(a.warmup):
+        eng.simulate(cfg, N, seed=1000 + i)
+    kernel_ms = []
+    events = None
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        r = eng.simulate(cfg, N, seed=1234 + i)
+        kernel_ms.append(r.kernel_ms)
+        if events is None:
+            events = r.events
+    elapsed = time.perf_counter() - t0
+    value = N * a.steps / elapsed
+    launch = eng.last_launch()
+
+    W = slots_per_packet(events, N)
+    avg_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+    achieved = W * N / avg_s / 1e12
+    peak = info["n_cu"] * 64 * info["clock_khz"] * 1e3 / 1e12
+    out = {
+        "metric": "photon packets/s on 1 GPU",
+        "value": value,
+        "unit": "packets/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "single-layer slab between glass slides, dual integrating spheres",
+                   "packets_per_step": N, "layers": "glass 1.52/0.1 | mu_a 1, mu_s 100, g 0.9, "
+                   "n 1.40, 0.1 cm | glass 1.52/0.1", "spheres": "R=4 cm, ports 0.5/0.5/0.25 cm, pw 0.98",
+                   "roulette": "1e-4, 1 in 10", "seed": 1234, "grid_blocks": launch["grid"],
+                   "block_threads": launch["block"]},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tslot/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "slots_per_packet": W, "kernel_ms_avg": avg_s * 1e3,
+                     "kernel_packets_per_s": N / avg_s,
+                     "events_per_packet": {k: v / N for k, v in events.items()}},
+    }
+    if not a.no_extra:
+        n3 = 100_000_000
+        r3 = eng.simulate(three_layer_sample(), n3, seed=77)
+        out["three_layer_packets_per_s"] = n3 / (r3.kernel_ms / 1e3)
+        n64 = 1 << 26
+        r64 = eng.simulate(cfg, n64, seed=78, fp64=True)
+        out["fp64_kernel_packets_per_s"] = n64 / (r64.kernel_ms / 1e3)
+    if not a.no_cpu:
+        print("bench: timing the CPU baseline (%.0f s)" % a.cpu_seconds, file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

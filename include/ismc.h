@@ -1,0 +1,153 @@
+/*
+ * libismc — MI355X (gfx950) integrating-sphere Monte Carlo photon transport.
+ *
+ * C ABI (plain pointers and sizes) loaded by ctypes from ismc_amd/.  Every
+ * entry point returns 0 on success or a negative status; it never exits
+ * the process.  The caller owns all host buffers; the context owns device
+ * memory.  One context per host thread; calls block until their results
+ * are on the host.
+ *
+ * Reference interfaces replaced (pdcook/ISMC, cited path:line):
+ *   - ismc_simulate        <- ISMLMC() host wrapper, CUDA/CUDAISMLMC.h:656-755,
+ *                             and the Python propagators ISMC(), ISMC.py:164-406,
+ *                             ISMLMC(), ISMLMC.py:101-347
+ *   - ismc_simulate_batch  <- the per-point launch loop of
+ *                             singleLayerSampleSpace, CUDA/ISMLMC-lookup.cu:153-421,
+ *                             and the per-evaluation launches of cost(),
+ *                             CUDA/ISMLMC-minimization.cu:89-131
+ *   - ismc_create/destroy  <- the process-global RNG state set-up
+ *                             (rand_set/globalDeviceStates, CUDA/CUDAISMLMC.h:19-25,
+ *                             670-683) and cudaSetDevice (:660)
+ *   - ismc_device_info     <- CUDABasicProperties, CUDA/CUDAISMLMC.h:28-38
+ *   - ismc_last_error      <- gpuAssert's message, CUDA/CUDAISMLMC.h:41-48
+ */
+#ifndef ISMC_H
+#define ISMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISMC_MAX_LAYERS 16
+
+/* status codes */
+#define ISMC_OK 0
+#define ISMC_EINVAL -1      /* invalid argument or configuration          */
+#define ISMC_EDEVICE -2     /* HIP runtime error (message in last_error)   */
+#define ISMC_ENOMEM -3      /* device or pinned host allocation failed     */
+#define ISMC_ERANGE -4      /* a size exceeds what one launch can tally    */
+
+/* flags */
+#define ISMC_FLAG_FP64 0x1u           /* FP64 transport (deviation reference)  */
+#define ISMC_FLAG_CUDA_ROULETTE 0x4u  /* CUDAISMLMC.h:275: int 1/m == 0, no survivor */
+
+/* One integrating sphere (or, when !present, the angle threshold that
+ * separates specular/unscattered from diffuse light; CUDAISMLMC.h:413-433). */
+typedef struct {
+    int32_t present;
+    int32_t _pad;
+    double pw;     /* wall reflectance                                    */
+    double fs;     /* sample port fraction                                */
+    double fp;     /* source (R) / optional (T) port fraction             */
+    double fd;     /* detector port fraction                              */
+    double f;      /* total port fraction, normally fs + fp + fd          */
+    double angle;  /* radians; used only when !present                    */
+} ismc_sphere;
+
+/* Sample between the spheres: layers in order from the illuminated face.
+ * mu_s == 0 marks a non-scattering glass slide (CUDAISMLMC.h:102,155).
+ * mu_t is taken as given, like the reference (normally mu_a + mu_s). */
+typedef struct {
+    int32_t n_layers;
+    int32_t specular_included;   /* beam 8.1 deg off normal (CUDAISMLMC.h:347-352) */
+    double n_ambient;            /* 1.0 in the reference                           */
+    double roulette_threshold;   /* 1e-4 (CUDAISMLMC.h:140), 1e-3 in ISMC.py:136  */
+    double roulette_m;           /* survivor weight factor, 10                     */
+    double n[ISMC_MAX_LAYERS];
+    double g[ISMC_MAX_LAYERS];
+    double t[ISMC_MAX_LAYERS];
+    double mu_a[ISMC_MAX_LAYERS];
+    double mu_s[ISMC_MAX_LAYERS];
+    double mu_t[ISMC_MAX_LAYERS];
+    ismc_sphere R;
+    ismc_sphere T;
+} ismc_config;
+
+/* Tallies, order A, R_d, R_s, T_d, T_u (CUDAISMLMC.h:754).  sum[k] is the
+ * sum over packets of the weight fraction that packet left in tally k;
+ * sumsq[k] the sum of squares (for the standard error).  Divide by
+ * n_packets for the reference's fractions. */
+typedef struct {
+    double sum[5];
+    double sumsq[5];
+    uint64_t events[5];   /* interactions, boundary hits, entrances,
+                             sphere decisions, re-injections            */
+    uint64_t n_packets;
+    uint64_t n_aborted;   /* packets stopped by the per-packet event cap */
+} ismc_tally;
+
+/* Launch shape; zero fields mean "use the context's autotuned default". */
+typedef struct {
+    int32_t blocks_per_cu;
+    int32_t chunk_packets;   /* packets per work-queue chunk */
+    int32_t use_graph;       /* replay through a captured hipGraph when shapes repeat */
+    int32_t _pad;
+} ismc_tune;
+
+typedef struct {
+    char name[256];
+    char arch[64];
+    int32_t n_cu;
+    int32_t clock_khz;
+    int64_t total_mem;
+    int32_t warp_size;
+    int32_t _pad;
+} ismc_device_info;
+
+typedef struct ismc_ctx ismc_ctx;
+
+int ismc_create(int device, ismc_ctx** out);
+void ismc_destroy(ismc_ctx* ctx);
+const char* ismc_last_error(const ismc_ctx* ctx);
+int ismc_device_info_get(ismc_ctx* ctx, ismc_device_info* out);
+
+/* Check a configuration without touching a GPU; the message (if msg is
+ * non-NULL) explains a rejection.  Returns 0 or ISMC_EINVAL / ISMC_ERANGE. */
+int ismc_validate_config(const ismc_config* cfg, char* msg, int32_t msg_len);
+
+/* One configuration, packets first_packet_id .. first_packet_id+n-1 of
+ * Philox stream (seed, stream).  Results do not depend on launch shape. */
+int ismc_simulate(ismc_ctx* ctx, const ismc_config* cfg, uint64_t n_packets, uint64_t seed,
+                  uint64_t first_packet_id, uint32_t stream, uint32_t flags, ismc_tally* out);
+
+/* Many configurations with uneven packet counts in one persistent launch.
+ * first_packet_id and streams may be NULL (0 and the point index). */
+int ismc_simulate_batch(ismc_ctx* ctx, const ismc_config* cfgs, const uint64_t* n_packets,
+                        const uint64_t* first_packet_id, const uint32_t* streams,
+                        int32_t n_points, uint64_t seed, uint32_t flags, const ismc_tune* tune,
+                        ismc_tally* out);
+
+/* Per-packet records {A, R_d, R_s, T_d, T_u, events} (n_packets x 6
+ * doubles) for one configuration: a debugging and parity aid. */
+int ismc_simulate_packets(ismc_ctx* ctx, const ismc_config* cfg, uint64_t n_packets,
+                          uint64_t seed, uint64_t first_packet_id, uint32_t stream,
+                          uint32_t flags, double* per_packet, ismc_tally* out);
+
+/* Device time (ms, hipEvents on the context's stream) of the last
+ * transport launch, and the launch shape it used. */
+int ismc_last_launch(ismc_ctx* ctx, float* kernel_ms, int32_t* grid_blocks,
+                     int32_t* block_threads, int64_t* n_chunks);
+
+/* Evaluate the device physics helpers on host inputs (parity aid):
+ * op 0 fresnel(a,b,c)->out[0], cos_t(out[1]); op 1 hg_cos(g=a,u=b);
+ * op 2 cos(2 pi a); op 3 -ln(a); op 4 entrance reflectance of cfg at mu_z=a
+ * (side R if a > 0 else T); op 5 spin: mu_z'(mu_z=a, g=b, u_hg=c, u_phi=d). */
+int ismc_eval_physics(ismc_ctx* ctx, int32_t op, const ismc_config* cfg, const double* in4,
+                      int32_t n, uint32_t flags, double* out2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

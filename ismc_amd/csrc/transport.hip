@@ -1,0 +1,947 @@
+// libismc — photon transport for MI355X (gfx950).
+//
+// One persistent kernel runs every packet history of a launch: a packet
+// enters the sample from one of the two integrating spheres, hops, drops
+// and spins through the layers (Henyey-Greenstein scattering, Fresnel
+// reflection/refraction at every interface, Russian roulette), leaves
+// into a sphere, and is either captured (specular/unscattered port or
+// diffuse wall) or returned to the sample, possibly from the other side.
+// Semantics follow CUDA/CUDAISMLMC.h:99-654 and ISMLMC.py:101-347 +
+// MLMC.py:76-202 of the reference; the structure is our own:
+//
+//  * Event machine.  Each loop iteration advances every lane by exactly
+//    one event of its packet: an entrance (Fresnel at the outer face), a
+//    flight that ends in an interaction (hop, drop, roulette, spin), or a
+//    flight that ends on an interface (Fresnel reflect / refract / exit).
+//    Lanes whose packet ends pick up a new packet at the top of the next
+//    iteration, so a wave never waits for its longest history.
+//  * Only z and mu_z are carried.  The slab is laterally infinite and
+//    every tally, boundary distance, Fresnel test and port test depends on
+//    z and mu_z alone; after a spin the new mu_z depends only on (mu_z,
+//    theta, cos phi).  x, y, mu_x, mu_y (and sin phi) are dead state in the
+//    reference and are not computed.
+//  * Random numbers: Philox4x32-10 (Salmon et al. SC'11), key = seed,
+//    counter = (packet id lo, packet id hi, event index, stream).  Each
+//    event draws one 4-word block.  Words are used as
+//      entrance:     w0 reflect test, w1 first step (or return draw), w2 new |mu_z|
+//      interaction:  w0 next step, w1 HG, w2 azimuth, w3 roulette
+//      interface:    w0 Fresnel test, w1 return draw, w2 new |mu_z| (on exit)
+//    so a history is a pure function of (seed, stream, packet id):
+//    results do not depend on launch shape or scheduling.
+//  * Work queue.  The host splits every configuration ("point") into chunks
+//    of packet ids.  Each wave takes chunks from a global atomic counter;
+//    the chunk's point table (layers + sphere constants) is copied into a
+//    per-wave LDS slot (two slots, so lanes can start the next chunk while
+//    others finish the last one).  The current layer's constants live in
+//    registers and are reloaded from LDS only when a packet crosses into
+//    another layer.
+//  * Tallies.  Per-packet sums are kept in the transport precision; at the
+//    end of a packet its A and exit weight (and their squares) are added as
+//    64-bit fixed point (2^-32 quantum, round to nearest) into a per-wave
+//    LDS accumulator of its chunk's slot; a drained slot is flushed with
+//    one 64-bit atomic per tally into the point's global slot.  Integer
+//    sums are exact and order independent, so tallies are bitwise
+//    reproducible; the host converts them to FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ismc.h"
+
+namespace ismc {
+
+constexpr int kMaxLayers = ISMC_MAX_LAYERS;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
+constexpr uint32_t kMaxEventsPerPacket = 1u << 22;
+constexpr int kSlotWords = 16;  // per point: sum[5], sumsq[5], events[5], aborted
+
+enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2 };
+enum : int32_t { PH_NEED = 0, PH_ENTER = 1, PH_FLIGHT = 2, PH_DONE = 3 };
+
+// Constants of one layer, derived on the host in FP64.
+template <typename R>
+struct alignas(16) Layer {
+  R zlo, zhi, n, step_k;     // step_k = ln2 / mu_t: s = -log2(u) * step_k
+  R absfrac, hg_a, hg_b, hg_c;  // mu_a/mu_t; 1+g^2; 1-g^2; 1-g
+  R hg_d, hg_e, mu_t;        // 2g; 1/(2g); mu_t
+  int32_t kind;
+};
+
+// Everything a packet needs about its configuration.
+template <typename R>
+struct alignas(16) Point {
+  int32_t n_layers;
+  int32_t first_layer[2];    // entered from the R side / T side
+  int32_t ent_glass[2];      // first layer on that side is a glass slide
+  int32_t cuda_roulette;
+  R n_amb, t_tot, roul_thr, roul_m;
+  R roul_p, mu0_z;
+  R cos_spec[2];             // |mu_z| >= cos_spec: specular (R) / unscattered (T)
+  R p_return[2];             // sphere return probability; -1 without a sphere
+  R ent_n2[2], ent_n3[2];    // entrance: first layer index, and the one behind a slide
+  Layer<R> layer[kMaxLayers];
+};
+
+struct alignas(16) Chunk {
+  uint32_t point;
+  uint32_t stream;
+  uint64_t first_pid;
+  uint32_t count;
+  uint32_t _pad;
+  uint64_t out_base;  // dense packet index of first_pid (per-packet output)
+};
+
+struct Args {
+  const void* points;        // Point<R>[n_points]
+  const Chunk* chunks;
+  uint32_t n_chunks;
+  uint32_t seed_lo, seed_hi;
+  uint32_t* chunk_counter;
+  unsigned long long* slots; // [n_points][kSlotWords]
+  double* per_packet;        // optional [n][6]
+};
+
+// ---------------------------------------------------------------------------
+// device math
+template <typename R> struct DM;
+template <> struct DM<float> {
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float nlog2(float u) { return -__builtin_amdgcn_logf(u); }
+  static __device__ __forceinline__ float cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
+  static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
+  static __device__ __forceinline__ float csign(float a, float b) { return __builtin_copysignf(a, b); }
+  static __device__ __forceinline__ unsigned long long fx(float v) {
+    return (unsigned long long)__builtin_rintf(v * 4294967296.0f);
+  }
+};
+template <> struct DM<double> {
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double sqrt(double x) { return __builtin_sqrt(x); }
+  static __device__ __forceinline__ double nlog2(double u) { return -::log2(u); }
+  static __device__ __forceinline__ double cos2pi(double u) { return ::cospi(2.0 * u); }
+  static __device__ __forceinline__ double abs(double x) { return __builtin_fabs(x); }
+  static __device__ __forceinline__ double csign(double a, double b) { return __builtin_copysign(a, b); }
+  static __device__ __forceinline__ unsigned long long fx(double v) {
+    return (unsigned long long)__builtin_rint(v * 4294967296.0);
+  }
+};
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// 24-bit uniform in (0, 1]: exact in float and double.
+template <typename R>
+__device__ __forceinline__ R u01(uint32_t x) {
+  return (R)(float)((x >> 8) + 1u) * (R)5.9604644775390625e-08;
+}
+
+// Unpolarised Fresnel reflectance for light going from index ni to nt at
+// direction cosine mu_z (fresnel_snell, CUDAISMLMC.h:79-96), written with
+// cosines instead of angles: rs = (ni ci - nt ct)/(ni ci + nt ct),
+// rp = (ni ct - nt ci)/(ni ct + nt ci), R = (rs^2 + rp^2)/2.  Same
+// near-normal shortcut (|mu_z| > 0.99999) and total-internal-reflection rule.
+// ct (the transmitted cosine, Snell) is returned for the refraction.
+template <typename R>
+__device__ __forceinline__ R fresnel(R ni, R nt, R mu_z, R& ct) {
+  const R ci = DM<R>::abs(mu_z);
+  const R eta = ni * DM<R>::rcp(nt);
+  const R st2 = eta * eta * ((R)1 - ci * ci);
+  ct = DM<R>::sqrt(fmax((R)0, (R)1 - st2));
+  const R a = ni * ci, b = nt * ct, c = ni * ct, d = nt * ci;
+  const R rs = (a - b) * DM<R>::rcp(a + b);
+  const R rp = (c - d) * DM<R>::rcp(c + d);
+  const R q = (nt - ni) * DM<R>::rcp(nt + ni);
+  R Rf = (R)0.5 * (rs * rs + rp * rp);
+  Rf = st2 >= (R)1 ? (R)1 : Rf;
+  Rf = ci > (R)0.99999 ? q * q : Rf;
+  return Rf;
+}
+
+// Entrance reflectance (incident_reflection, CUDAISMLMC.h:99-122): through a
+// glass slide r1 + (1-r1)^2 r2 / (1 - r1 r2) with both Fresnel terms at the
+// air-side cosine; otherwise one air/sample interface.
+template <typename R>
+__device__ __forceinline__ R entrance_R(const Point<R>& P, int side, R mu_z, R& ct1) {
+  R ct2;
+  const R r1 = fresnel<R>(P.n_amb, P.ent_n2[side], mu_z, ct1);
+  const R r2 = fresnel<R>(P.ent_n2[side], P.ent_n3[side], mu_z, ct2);
+  const R rg = r1 + ((R)1 - r1) * ((R)1 - r1) * r2 * DM<R>::rcp((R)1 - r1 * r2);
+  return P.ent_glass[side] ? rg : r1;
+}
+
+// Henyey-Greenstein cos(theta) (CUDAISMLMC.h:284-285).
+template <typename R>
+__device__ __forceinline__ R hg_cos(int kind, R ha, R hb, R hc, R hd, R he, R u) {
+  const R q = hb * DM<R>::rcp(hc + hd * u);
+  const R c = (ha - q * q) * he;
+  return kind == KIND_ISO ? (R)2 * u - (R)1 : c;
+}
+
+// New mu_z after a scattering by theta with azimuth 2 pi u_phi
+// (CUDAISMLMC.h:287-308, z component).
+template <typename R>
+__device__ __forceinline__ R spin_z(R mu_z, R cos_t, R u_phi) {
+  const R cphi = DM<R>::cos2pi(u_phi);
+  const R st_sz = DM<R>::sqrt(fmax((R)0, ((R)1 - cos_t * cos_t) * ((R)1 - mu_z * mu_z)));
+  const R general = mu_z * cos_t - st_sz * cphi;
+  // near-vertical: mu_z' = sign(mu_z) * cos_t (cos_t keeps its own sign)
+  return DM<R>::abs(mu_z) > (R)0.99999 ? (mu_z > (R)0 ? cos_t : -cos_t) : general;
+}
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <typename R>
+__device__ __forceinline__ void copy_point(Point<R>* dst, const Point<R>* src, int lane) {
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  constexpr int words = (int)(sizeof(Point<R>) / 4);
+  for (int i = lane; i < words; i += 64) d[i] = s[i];
+}
+
+template <typename R>
+__device__ __forceinline__ void flush_slot(unsigned long long* acc, unsigned long long* slots,
+                                           uint32_t point, int lane) {
+  if (lane < kSlotWords) {
+    const unsigned long long v = acc[lane];
+    if (v) atomicAdd(&slots[(size_t)point * kSlotWords + lane], v);
+    acc[lane] = 0ull;
+  }
+}
+
+template <typename R, bool kPerPacket>
+__global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
+  __shared__ Point<R> s_pt[kWavesPerBlock][2];
+  __shared__ unsigned long long s_acc[kWavesPerBlock][2][kSlotWords];
+
+  const int wid = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  Point<R>* pts = s_pt[wid];
+  unsigned long long* acc0 = s_acc[wid][0];
+  unsigned long long* acc1 = s_acc[wid][1];
+  if (lane < kSlotWords) { acc0[lane] = 0ull; acc1[lane] = 0ull; }
+  __builtin_amdgcn_wave_barrier();
+
+  const Point<R>* gpts = reinterpret_cast<const Point<R>*>(a.points);
+  const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+
+  // wave-uniform queue state
+  int cur_set = 1;
+  uint32_t set_point0 = 0xffffffffu, set_point1 = 0xffffffffu;
+  uint32_t outstanding0 = 0, outstanding1 = 0;
+  uint64_t win_next = 0, win_end = 0, win_first = 0, win_outbase = 0;
+  uint32_t win_stream = 0;
+  bool exhausted = false;
+
+  // lane state
+  int phase = PH_NEED, side = 0, layer = 0, set = 0;
+  R w = 0, z = 0, muz = 1, s = 0, A = 0;
+  uint64_t pid = 0, outidx = 0;
+  uint32_t ev = 0, stream = 0;
+  uint32_t e_int = 0, e_bnd = 0, e_inc = 0, e_sph = 0, e_rei = 0;
+  // current layer cache
+  R L_zlo = 0, L_zhi = 0, L_n = 1, L_step = 0, L_abs = 0, L_ha = 0, L_hb = 0, L_hc = 0,
+    L_hd = 0, L_he = 0, L_mut = 0;
+  int L_kind = KIND_GLASS;
+
+  for (;;) {
+    // ---------------- refill idle lanes from the wave's window ----------------
+    uint64_t need = __ballot(phase == PH_NEED);
+    while (need) {
+      if (win_next == win_end) {
+        if (exhausted) {
+          if (phase == PH_NEED) phase = PH_DONE;
+          break;
+        }
+        const int ns = cur_set ^ 1;
+        const uint32_t busy = ns ? outstanding1 : outstanding0;
+        if (busy) break;  // lanes of the chunk held in that slot still run
+        const uint32_t sp = ns ? set_point1 : set_point0;
+        if (sp != 0xffffffffu) flush_slot<R>(ns ? acc1 : acc0, a.slots, sp, lane);
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(a.chunk_counter, 1u);
+        k = __builtin_amdgcn_readfirstlane(k);
+        if (k >= a.n_chunks) {
+          exhausted = true;
+          if (ns) set_point1 = 0xffffffffu; else set_point0 = 0xffffffffu;
+          continue;
+        }
+        const Chunk ch = a.chunks[k];
+        copy_point<R>(&pts[ns], &gpts[ch.point], lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (ns) set_point1 = ch.point; else set_point0 = ch.point;
+        cur_set = ns;
+        win_next = ch.first_pid;
+        win_end = ch.first_pid + ch.count;
+        win_first = ch.first_pid;
+        win_outbase = ch.out_base;
+        win_stream = ch.stream;
+      }
+      const uint64_t avail = win_end - win_next;
+      const uint32_t nneed = (uint32_t)__popcll(need);
+      const uint32_t take = avail < nneed ? (uint32_t)avail : nneed;
+      const uint32_t rank = lane_rank(need);
+      if (phase == PH_NEED && rank < take) {
+        pid = win_next + rank;
+        set = cur_set;
+        stream = win_stream;
+        outidx = win_outbase + (pid - win_first);
+        w = (R)1;
+        muz = pts[set].mu0_z;
+        side = 0;
+        A = (R)0;
+        ev = 0;
+        phase = PH_ENTER;
+      }
+      win_next += take;
+      if (cur_set) outstanding1 += take; else outstanding0 += take;
+      need = __ballot(phase == PH_NEED);
+    }
+    if (__ballot(phase == PH_ENTER || phase == PH_FLIGHT) == 0) {
+      if (__ballot(phase != PH_DONE) == 0) break;
+      continue;
+    }
+
+    // ---------------- one event per active lane ----------------
+    const bool active = (phase == PH_ENTER || phase == PH_FLIGHT);
+    uint32_t rw[4];
+    philox((uint32_t)pid, (uint32_t)(pid >> 32), ev, stream, k0, k1, rw);
+    const R u0 = u01<R>(rw[0]), u1 = u01<R>(rw[1]), u2 = u01<R>(rw[2]), u3 = u01<R>(rw[3]);
+    ev += 1;
+    const Point<R>& P = pts[set];
+
+    bool to_sphere = false, ended = false, aborted = false;
+    int term = 0;  // tally index of the exit weight (0 = none)
+
+    if (phase == PH_FLIGHT) {
+      const R zb = muz > (R)0 ? L_zhi : L_zlo;
+      const R dz = zb - z;
+      const bool glass = (L_kind == KIND_GLASS);
+      const bool hit = glass || (muz != (R)0 && s * DM<R>::abs(muz) >= DM<R>::abs(dz));
+      if (!hit) {
+        // hop, drop, roulette, spin (CUDAISMLMC.h:259-309)
+        z += s * muz;
+        const R dW = w * L_abs;
+        w -= dW;
+        A += dW;
+        e_int += 1;
+        if (w <= P.roul_thr) {
+          const bool survive = !P.cuda_roulette && (u3 <= P.roul_p);
+          w = survive ? w * P.roul_m : (R)0;
+        }
+        if (w > (R)0) {
+          const R ct = hg_cos<R>(L_kind, L_ha, L_hb, L_hc, L_hd, L_he, u1);
+          muz = spin_z<R>(muz, ct, u2);
+          s = DM<R>::nlog2(u0) * L_step;
+        } else {
+          ended = true;
+        }
+      } else {
+        // move to the interface (CUDAISMLMC.h:180-257)
+        const R d = dz * DM<R>::rcp(muz);
+        z = zb;
+        s = glass ? (R)0 : s - d;
+        const int nxt = muz > (R)0 ? layer + 1 : layer - 1;
+        const bool outside = (nxt < 0) || (nxt >= P.n_layers);
+        const int nxc = outside ? layer : nxt;
+        const R nn = outside ? P.n_amb : P.layer[nxc].n;
+        R ct;
+        const R Rf = fresnel<R>(L_n, nn, muz, ct);
+        e_bnd += 1;
+        if (u0 < Rf) {
+          muz = -muz;
+        } else {
+          muz = DM<R>::csign(ct, muz);
+          if (outside) {
+            side = nxt < 0 ? 0 : 1;
+            to_sphere = true;
+          } else {
+            const Layer<R>& NL = P.layer[nxt];
+            if (NL.kind != KIND_GLASS && s != (R)0) s *= L_mut * DM<R>::rcp(NL.mu_t);
+            layer = nxt;
+            L_zlo = NL.zlo; L_zhi = NL.zhi; L_n = NL.n; L_step = NL.step_k; L_abs = NL.absfrac;
+            L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
+            L_mut = NL.mu_t; L_kind = NL.kind;
+          }
+        }
+      }
+    } else if (phase == PH_ENTER) {
+      // (re-)entrance through the outer face (CUDAISMLMC.h:367-369, 502-503)
+      e_inc += 1;
+      R ct;
+      const R Re = entrance_R<R>(P, side, muz, ct);
+      if (u0 < Re) {
+        to_sphere = true;
+      } else {
+        muz = DM<R>::csign(ct, muz);
+        layer = P.first_layer[side];
+        const Layer<R>& NL = P.layer[layer];
+        L_zlo = NL.zlo; L_zhi = NL.zhi; L_n = NL.n; L_step = NL.step_k; L_abs = NL.absfrac;
+        L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
+        L_mut = NL.mu_t; L_kind = NL.kind;
+        z = side ? P.t_tot : (R)0;
+        s = L_kind == KIND_GLASS ? (R)0 : DM<R>::nlog2(u1) * L_step;
+        phase = PH_FLIGHT;
+      }
+    }
+    if (to_sphere) {
+      // sphere capture / return (CUDAISMLMC.h:377-410, 442-475, 518-550, 583-615)
+      e_sph += 1;
+      if (DM<R>::abs(muz) >= P.cos_spec[side]) {
+        term = side ? 4 : 2;
+        ended = true;
+      } else if (u1 < P.p_return[side]) {
+        muz = side ? -u2 : u2;
+        phase = PH_ENTER;
+        e_rei += 1;
+      } else {
+        term = side ? 3 : 1;
+        ended = true;
+      }
+    }
+    if (active && !ended && ev > kMaxEventsPerPacket) {
+      ended = true;
+      aborted = true;
+    }
+
+    if (ended) {
+      unsigned long long* acc = set ? acc1 : acc0;
+      if (aborted) {
+        atomicAdd(&acc[15], 1ull);
+        A = (R)0;
+        term = 0;
+      } else {
+        atomicAdd(&acc[0], DM<R>::fx(A));
+        atomicAdd(&acc[5], DM<R>::fx(A * A));
+        if (term) {
+          atomicAdd(&acc[term], DM<R>::fx(w));
+          atomicAdd(&acc[5 + term], DM<R>::fx(w * w));
+        }
+      }
+      atomicAdd(&acc[10], (unsigned long long)e_int);
+      atomicAdd(&acc[11], (unsigned long long)e_bnd);
+      atomicAdd(&acc[12], (unsigned long long)e_inc);
+      atomicAdd(&acc[13], (unsigned long long)e_sph);
+      atomicAdd(&acc[14], (unsigned long long)e_rei);
+      if (kPerPacket) {
+        double* o = a.per_packet + outidx * 6;
+        o[0] = (double)A;
+        o[1] = term == 1 ? (double)w : 0.0;
+        o[2] = term == 2 ? (double)w : 0.0;
+        o[3] = term == 3 ? (double)w : 0.0;
+        o[4] = term == 4 ? (double)w : 0.0;
+        o[5] = (double)ev;
+      }
+      e_int = e_bnd = e_inc = e_sph = e_rei = 0;
+      phase = PH_NEED;
+    }
+    const uint64_t end0 = __ballot(ended && set == 0);
+    const uint64_t end1 = __ballot(ended && set == 1);
+    outstanding0 -= (uint32_t)__popcll(end0);
+    outstanding1 -= (uint32_t)__popcll(end1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (set_point0 != 0xffffffffu) flush_slot<R>(acc0, a.slots, set_point0, lane);
+  if (set_point1 != 0xffffffffu) flush_slot<R>(acc1, a.slots, set_point1, lane);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, size_t n, uint32_t* counter) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+  if (i == 0) *counter = 0u;
+}
+
+// Device helpers on host inputs (parity aid for ismc_eval_physics).
+template <typename R>
+__global__ void eval_kernel(int op, const Point<R>* pt, const double* in4, int n, double* out2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* x = in4 + 4 * i;
+  double o0 = 0, o1 = 0;
+  if (op == 0) {
+    R ct;
+    o0 = (double)fresnel<R>((R)x[0], (R)x[1], (R)x[2], ct);
+    o1 = (double)ct;
+  } else if (op == 1) {
+    const R g = (R)x[0];
+    const int kind = g == (R)0 ? KIND_ISO : KIND_HG;
+    o0 = (double)hg_cos<R>(kind, (R)1 + g * g, (R)1 - g * g, (R)1 - g, (R)2 * g,
+                           g == (R)0 ? (R)0 : (R)1 / ((R)2 * g), (R)x[1]);
+  } else if (op == 2) {
+    o0 = (double)DM<R>::cos2pi((R)x[0]);
+  } else if (op == 3) {
+    o0 = (double)(DM<R>::nlog2((R)x[0]) * (R)0.6931471805599453);
+  } else if (op == 4) {
+    R ct;
+    const R mz = (R)x[0];
+    o0 = (double)entrance_R<R>(*pt, mz > (R)0 ? 0 : 1, mz, ct);
+    o1 = (double)ct;
+  } else if (op == 5) {
+    const R g = (R)x[1];
+    const int kind = g == (R)0 ? KIND_ISO : KIND_HG;
+    const R c = hg_cos<R>(kind, (R)1 + g * g, (R)1 - g * g, (R)1 - g, (R)2 * g,
+                          g == (R)0 ? (R)0 : (R)1 / ((R)2 * g), (R)x[2]);
+    o0 = (double)spin_z<R>((R)x[0], c, (R)x[3]);
+    o1 = (double)c;
+  }
+  out2[2 * i] = o0;
+  out2[2 * i + 1] = o1;
+}
+
+}  // namespace ismc
+
+// ===========================================================================
+// host side
+// ===========================================================================
+using namespace ismc;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct ismc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop{};
+  std::string err;
+  DevBuf points, chunks, slots, counter, per_packet, eval_in, eval_out;
+  void* h_stage = nullptr;  // pinned staging
+  size_t h_stage_cap = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float last_ms = 0.f;
+  int32_t last_grid = 0;
+  int64_t last_chunks = 0;
+  int blocks_per_cu = 8;
+  int chunk_packets = 4096;
+};
+
+static int set_err(ismc_ctx* c, int code, const char* fmt, const char* a = "", long long b = 0) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), fmt, a, b);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHECK(ctx, call)                                                              \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return set_err(ctx, ISMC_EDEVICE, "%s (line %lld)", hipGetErrorString(e_), __LINE__); \
+  } while (0)
+
+static int ensure_dev(ismc_ctx* c, DevBuf& b, size_t bytes) {
+  if (b.cap >= bytes && b.p) return 0;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    (void)e;
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  size_t want = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return set_err(c, ISMC_ENOMEM, "hipMalloc(%s) failed for %lld bytes", hipGetErrorString(e),
+                   (long long)want);
+  }
+  b.cap = want;
+  return 0;
+}
+
+static int ensure_stage(ismc_ctx* c, size_t bytes) {
+  if (c->h_stage_cap >= bytes) return 0;
+  if (c->h_stage) {
+    hipError_t e = hipHostFree(c->h_stage);
+    (void)e;
+    c->h_stage = nullptr;
+    c->h_stage_cap = 0;
+  }
+  size_t want = std::max<size_t>(bytes, 1 << 16);
+  hipError_t e = hipHostMalloc(&c->h_stage, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    c->h_stage = nullptr;
+    return set_err(c, ISMC_ENOMEM, "hipHostMalloc(%s) failed for %lld bytes", hipGetErrorString(e),
+                   (long long)want);
+  }
+  c->h_stage_cap = want;
+  return 0;
+}
+
+static bool finite_pos(double v) { return std::isfinite(v) && v > 0; }
+
+static int validate(ismc_ctx* c, const ismc_config* cfg) {
+  if (!cfg) return set_err(c, ISMC_EINVAL, "null config%s", "");
+  if (cfg->n_layers < 1 || cfg->n_layers > ISMC_MAX_LAYERS)
+    return set_err(c, ISMC_EINVAL, "n_layers must be in 1..16 (got %s%lld)", "", cfg->n_layers);
+  if (!finite_pos(cfg->n_ambient)) return set_err(c, ISMC_EINVAL, "n_ambient must be > 0%s");
+  if (!(cfg->roulette_threshold >= 0) || !(cfg->roulette_m >= 1))
+    return set_err(c, ISMC_EINVAL, "roulette_threshold >= 0 and roulette_m >= 1 required%s");
+  if (cfg->roulette_threshold * cfg->roulette_m > 1.0)
+    return set_err(c, ISMC_ERANGE, "roulette_threshold * roulette_m must be <= 1%s");
+  int turbid = 0;
+  for (int i = 0; i < cfg->n_layers; ++i) {
+    if (!finite_pos(cfg->n[i]) || !finite_pos(cfg->t[i]))
+      return set_err(c, ISMC_EINVAL, "layer %s%lld: n and t must be finite and > 0", "", i);
+    if (!(cfg->mu_a[i] >= 0) || !(cfg->mu_s[i] >= 0) || !std::isfinite(cfg->mu_a[i]) ||
+        !std::isfinite(cfg->mu_s[i]))
+      return set_err(c, ISMC_EINVAL, "layer %s%lld: mu_a, mu_s must be finite and >= 0", "", i);
+    if (!(cfg->g[i] > -1.0 && cfg->g[i] < 1.0) && cfg->mu_s[i] != 0.0)
+      return set_err(c, ISMC_EINVAL, "layer %s%lld: g must be in (-1, 1)", "", i);
+    if (cfg->mu_s[i] != 0.0) {
+      if (!finite_pos(cfg->mu_t[i]))
+        return set_err(c, ISMC_EINVAL, "layer %s%lld: mu_t must be > 0", "", i);
+      ++turbid;
+    }
+  }
+  if (!turbid && cfg->n_layers == 1 && cfg->mu_s[0] == 0.0)
+    return set_err(c, ISMC_EINVAL, "a sample needs at least one scattering layer%s");
+  const ismc_sphere* sp[2] = {&cfg->R, &cfg->T};
+  for (int k = 0; k < 2; ++k) {
+    if (sp[k]->present) {
+      const double den = 1.0 - sp[k]->pw * (1.0 - sp[k]->f);
+      if (!(sp[k]->fp >= 0 && sp[k]->fp <= 1) || !(den > 0) || !std::isfinite(sp[k]->pw) ||
+          !std::isfinite(sp[k]->fs))
+        return set_err(c, ISMC_EINVAL, "sphere %s: need 0 <= fp <= 1 and pw (1-f) < 1",
+                       k ? "T" : "R");
+    } else if (!std::isfinite(sp[k]->angle)) {
+      return set_err(c, ISMC_EINVAL, "sphere %s: angle must be finite", k ? "T" : "R");
+    }
+  }
+  return 0;
+}
+
+template <typename R>
+static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
+  memset(P, 0, sizeof(*P));
+  const int L = cfg->n_layers;
+  P->n_layers = L;
+  P->cuda_roulette = (flags & ISMC_FLAG_CUDA_ROULETTE) ? 1 : 0;
+  double z = 0;
+  for (int i = 0; i < L; ++i) {
+    Layer<R>& Ly = P->layer[i];
+    const double mu_t = cfg->mu_t[i], g = cfg->g[i];
+    Ly.zlo = (R)z;
+    z += cfg->t[i];
+    Ly.zhi = (R)z;
+    Ly.n = (R)cfg->n[i];
+    if (cfg->mu_s[i] == 0.0) {
+      Ly.kind = KIND_GLASS;
+    } else {
+      Ly.kind = g == 0.0 ? KIND_ISO : KIND_HG;
+      Ly.step_k = (R)(std::log(2.0) / mu_t);
+      Ly.absfrac = (R)(cfg->mu_a[i] / mu_t);
+      Ly.hg_a = (R)(1 + g * g);
+      Ly.hg_b = (R)(1 - g * g);
+      Ly.hg_c = (R)(1 - g);
+      Ly.hg_d = (R)(2 * g);
+      Ly.hg_e = g == 0.0 ? (R)0 : (R)(1 / (2 * g));
+      Ly.mu_t = (R)mu_t;
+    }
+  }
+  P->t_tot = (R)z;
+  P->n_amb = (R)cfg->n_ambient;
+  P->roul_thr = (R)cfg->roulette_threshold;
+  P->roul_m = (R)cfg->roulette_m;
+  P->roul_p = (R)(1.0 / cfg->roulette_m);
+  P->mu0_z = cfg->specular_included ? (R)0.99 : (R)1;
+  const int first[2] = {0, L - 1};
+  const int behind[2] = {1, L - 2};
+  const ismc_sphere* sp[2] = {&cfg->R, &cfg->T};
+  for (int k = 0; k < 2; ++k) {
+    P->first_layer[k] = first[k];
+    P->ent_glass[k] = cfg->mu_s[first[k]] == 0.0;
+    P->ent_n2[k] = (R)cfg->n[first[k]];
+    P->ent_n3[k] = (behind[k] >= 0 && behind[k] < L) ? (R)cfg->n[behind[k]] : (R)cfg->n_ambient;
+    if (sp[k]->present) {
+      P->cos_spec[k] = (R)std::sqrt(1.0 - sp[k]->fp);
+      P->p_return[k] = (R)((sp[k]->pw * sp[k]->fs) / (1.0 - sp[k]->pw * (1.0 - sp[k]->f)));
+    } else {
+      const double fp = 0.5 * (1.0 - std::cos(2.0 * sp[k]->angle));
+      P->cos_spec[k] = (R)std::sqrt(1.0 - fp);
+      P->p_return[k] = (R)-1;
+    }
+  }
+}
+
+template <typename R>
+static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
+                      const uint64_t* first, const uint32_t* streams, int32_t n_points,
+                      uint64_t seed, uint32_t flags, const ismc_tune* tune, ismc_tally* out,
+                      double* per_packet_host) {
+  const int chunk = (tune && tune->chunk_packets > 0) ? tune->chunk_packets : c->chunk_packets;
+  const int bpc = (tune && tune->blocks_per_cu > 0) ? tune->blocks_per_cu : c->blocks_per_cu;
+  // chunk list
+  std::vector<Chunk> chunks;
+  uint64_t total = 0;
+  for (int p = 0; p < n_points; ++p) {
+    if (npk[p] > (1ull << 32))
+      return set_err(c, ISMC_ERANGE, "point %s%lld: more than 2^32 packets in one call", "", p);
+    const uint64_t f0 = first ? first[p] : 0ull;
+    if (f0 + npk[p] < f0) return set_err(c, ISMC_ERANGE, "packet id range overflows%s");
+    for (uint64_t k = 0; k < npk[p]; k += (uint64_t)chunk) {
+      Chunk ch{};
+      ch.point = (uint32_t)p;
+      ch.stream = streams ? streams[p] : (uint32_t)p;
+      ch.first_pid = f0 + k;
+      ch.count = (uint32_t)std::min<uint64_t>((uint64_t)chunk, npk[p] - k);
+      ch.out_base = total + k;
+      chunks.push_back(ch);
+    }
+    total += npk[p];
+  }
+  if (chunks.size() > 0xffffffffull) return set_err(c, ISMC_ERANGE, "too many chunks%s");
+  const size_t pbytes = sizeof(Point<R>) * (size_t)n_points;
+  const size_t cbytes = sizeof(Chunk) * std::max<size_t>(chunks.size(), 1);
+  const size_t sbytes = sizeof(unsigned long long) * kSlotWords * (size_t)n_points;
+  int rc;
+  if ((rc = ensure_dev(c, c->points, pbytes))) return rc;
+  if ((rc = ensure_dev(c, c->chunks, cbytes))) return rc;
+  if ((rc = ensure_dev(c, c->slots, sbytes))) return rc;
+  if ((rc = ensure_dev(c, c->counter, 64))) return rc;
+  if ((rc = ensure_stage(c, pbytes + cbytes + sbytes))) return rc;
+  char* st = (char*)c->h_stage;
+  Point<R>* hp = reinterpret_cast<Point<R>*>(st);
+  for (int p = 0; p < n_points; ++p) compile_point<R>(&cfgs[p], flags, &hp[p]);
+  memcpy(st + pbytes, chunks.data(), sizeof(Chunk) * chunks.size());
+  HIPCHECK(c, hipMemcpyAsync(c->points.p, st, pbytes, hipMemcpyHostToDevice, c->stream));
+  if (!chunks.empty())
+    HIPCHECK(c, hipMemcpyAsync(c->chunks.p, st + pbytes, sizeof(Chunk) * chunks.size(),
+                               hipMemcpyHostToDevice, c->stream));
+  const size_t nslot = (size_t)kSlotWords * n_points;
+  zero_u64_kernel<<<(unsigned)((nslot + 255) / 256), 256, 0, c->stream>>>(
+      (unsigned long long*)c->slots.p, nslot, (uint32_t*)c->counter.p);
+  HIPCHECK(c, hipGetLastError());
+  if (per_packet_host) {
+    if ((rc = ensure_dev(c, c->per_packet, sizeof(double) * 6 * std::max<uint64_t>(total, 1))))
+      return rc;
+  }
+
+  Args a{};
+  a.points = c->points.p;
+  a.chunks = (const Chunk*)c->chunks.p;
+  a.n_chunks = (uint32_t)chunks.size();
+  a.seed_lo = (uint32_t)seed;
+  a.seed_hi = (uint32_t)(seed >> 32);
+  a.chunk_counter = (uint32_t*)c->counter.p;
+  a.slots = (unsigned long long*)c->slots.p;
+  a.per_packet = per_packet_host ? (double*)c->per_packet.p : nullptr;
+
+  // persistent grid: as many blocks as stay resident (occupancy), unless the
+  // caller asks for a given number per CU
+  int resident = 0;
+  if (per_packet_host)
+    HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, true>, kBlock, 0));
+  else
+    HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, false>, kBlock, 0));
+  const int per_cu = (tune && tune->blocks_per_cu > 0) ? bpc : std::max(1, resident);
+  const uint64_t waves_useful = std::max<uint64_t>(1, (total + 63) / 64);
+  const uint64_t blocks_useful = (waves_useful + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int grid = (int)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)c->prop.multiProcessorCount * per_cu, blocks_useful));
+  HIPCHECK(c, hipEventRecord(c->e0, c->stream));
+  if (per_packet_host)
+    transport_kernel<R, true><<<grid, kBlock, 0, c->stream>>>(a);
+  else
+    transport_kernel<R, false><<<grid, kBlock, 0, c->stream>>>(a);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipEventRecord(c->e1, c->stream));
+  unsigned long long* hs = reinterpret_cast<unsigned long long*>(st + pbytes + cbytes);
+  HIPCHECK(c, hipMemcpyAsync(hs, c->slots.p, sbytes, hipMemcpyDeviceToHost, c->stream));
+  if (per_packet_host)
+    HIPCHECK(c, hipMemcpyAsync(per_packet_host, c->per_packet.p, sizeof(double) * 6 * total,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipEventElapsedTime(&c->last_ms, c->e0, c->e1));
+  c->last_grid = grid;
+  c->last_chunks = (int64_t)chunks.size();
+  const double inv = 1.0 / 4294967296.0;
+  for (int p = 0; p < n_points; ++p) {
+    const unsigned long long* v = hs + (size_t)p * kSlotWords;
+    ismc_tally& t = out[p];
+    for (int k = 0; k < 5; ++k) {
+      t.sum[k] = (double)v[k] * inv;
+      t.sumsq[k] = (double)v[5 + k] * inv;
+      t.events[k] = v[10 + k];
+    }
+    t.n_aborted = v[15];
+    t.n_packets = npk[p];
+  }
+  return 0;
+}
+
+static int run_dispatch(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
+                        const uint64_t* first, const uint32_t* streams, int32_t n_points,
+                        uint64_t seed, uint32_t flags, const ismc_tune* tune, ismc_tally* out,
+                        double* per_packet_host) {
+  if (!c) return ISMC_EINVAL;
+  if (!cfgs || !npk || !out || n_points < 1) return set_err(c, ISMC_EINVAL, "null argument%s");
+  for (int p = 0; p < n_points; ++p) {
+    int rc = validate(c, &cfgs[p]);
+    if (rc) return rc;
+  }
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (flags & ISMC_FLAG_FP64)
+    return run_points<double>(c, cfgs, npk, first, streams, n_points, seed, flags, tune, out,
+                              per_packet_host);
+  return run_points<float>(c, cfgs, npk, first, streams, n_points, seed, flags, tune, out,
+                           per_packet_host);
+}
+
+extern "C" {
+
+int ismc_create(int device, ismc_ctx** out) {
+  if (!out) return ISMC_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return ISMC_EDEVICE;
+  if (device < 0 || device >= n) return ISMC_EINVAL;
+  ismc_ctx* c = new ismc_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&c->prop, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
+    delete c;
+    return ISMC_EDEVICE;
+  }
+  *out = c;
+  return 0;
+}
+
+void ismc_destroy(ismc_ctx* c) {
+  if (!c) return;
+  hipError_t e = hipSetDevice(c->device);
+  if (c->stream) e = hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->points, &c->chunks, &c->slots, &c->counter, &c->per_packet, &c->eval_in,
+                    &c->eval_out};
+  for (DevBuf* b : bufs)
+    if (b->p) e = hipFree(b->p);
+  if (c->h_stage) e = hipHostFree(c->h_stage);
+  if (c->e0) e = hipEventDestroy(c->e0);
+  if (c->e1) e = hipEventDestroy(c->e1);
+  if (c->stream) e = hipStreamDestroy(c->stream);
+  (void)e;
+  delete c;
+}
+
+const char* ismc_last_error(const ismc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ismc_validate_config(const ismc_config* cfg, char* msg, int32_t msg_len) {
+  ismc_ctx tmp;
+  const int rc = validate(&tmp, cfg);
+  if (msg && msg_len > 0) snprintf(msg, (size_t)msg_len, "%s", tmp.err.c_str());
+  return rc;
+}
+
+int ismc_device_info_get(ismc_ctx* c, ismc_device_info* out) {
+  if (!c || !out) return ISMC_EINVAL;
+  memset(out, 0, sizeof(*out));
+  snprintf(out->name, sizeof(out->name), "%s", c->prop.name);
+  snprintf(out->arch, sizeof(out->arch), "%s", c->prop.gcnArchName);
+  out->n_cu = c->prop.multiProcessorCount;
+  out->clock_khz = c->prop.clockRate;
+  out->total_mem = (int64_t)c->prop.totalGlobalMem;
+  out->warp_size = c->prop.warpSize;
+  return 0;
+}
+
+int ismc_simulate(ismc_ctx* c, const ismc_config* cfg, uint64_t n_packets, uint64_t seed,
+                  uint64_t first_packet_id, uint32_t stream, uint32_t flags, ismc_tally* out) {
+  return run_dispatch(c, cfg, &n_packets, &first_packet_id, &stream, 1, seed, flags, nullptr, out,
+                      nullptr);
+}
+
+int ismc_simulate_batch(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* n_packets,
+                        const uint64_t* first_packet_id, const uint32_t* streams,
+                        int32_t n_points, uint64_t seed, uint32_t flags, const ismc_tune* tune,
+                        ismc_tally* out) {
+  return run_dispatch(c, cfgs, n_packets, first_packet_id, streams, n_points, seed, flags, tune,
+                      out, nullptr);
+}
+
+int ismc_simulate_packets(ismc_ctx* c, const ismc_config* cfg, uint64_t n_packets, uint64_t seed,
+                          uint64_t first_packet_id, uint32_t stream, uint32_t flags,
+                          double* per_packet, ismc_tally* out) {
+  if (!per_packet) return set_err(c, ISMC_EINVAL, "null per_packet buffer%s");
+  if (n_packets > (1ull << 26)) return set_err(c, ISMC_ERANGE, "per-packet output limited to 2^26 packets%s");
+  return run_dispatch(c, cfg, &n_packets, &first_packet_id, &stream, 1, seed, flags, nullptr, out,
+                      per_packet);
+}
+
+int ismc_last_launch(ismc_ctx* c, float* kernel_ms, int32_t* grid_blocks, int32_t* block_threads,
+                     int64_t* n_chunks) {
+  if (!c) return ISMC_EINVAL;
+  if (kernel_ms) *kernel_ms = c->last_ms;
+  if (grid_blocks) *grid_blocks = c->last_grid;
+  if (block_threads) *block_threads = kBlock;
+  if (n_chunks) *n_chunks = c->last_chunks;
+  return 0;
+}
+
+int ismc_eval_physics(ismc_ctx* c, int32_t op, const ismc_config* cfg, const double* in4,
+                      int32_t n, uint32_t flags, double* out2) {
+  if (!c || !in4 || !out2 || n < 0 || op < 0 || op > 5) return set_err(c, ISMC_EINVAL, "bad eval_physics arguments%s");
+  if (op == 4) {
+    int rc = validate(c, cfg);
+    if (rc) return rc;
+  }
+  if (n == 0) return 0;
+  HIPCHECK(c, hipSetDevice(c->device));
+  int rc;
+  const bool f64 = flags & ISMC_FLAG_FP64;
+  const size_t pbytes = f64 ? sizeof(Point<double>) : sizeof(Point<float>);
+  if ((rc = ensure_dev(c, c->eval_in, sizeof(double) * 4 * (size_t)n))) return rc;
+  if ((rc = ensure_dev(c, c->eval_out, sizeof(double) * 2 * (size_t)n))) return rc;
+  if ((rc = ensure_dev(c, c->points, pbytes))) return rc;
+  if ((rc = ensure_stage(c, pbytes))) return rc;
+  if (cfg) {
+    if (f64) compile_point<double>(cfg, flags, (Point<double>*)c->h_stage);
+    else compile_point<float>(cfg, flags, (Point<float>*)c->h_stage);
+  } else {
+    memset(c->h_stage, 0, pbytes);
+  }
+  HIPCHECK(c, hipMemcpyAsync(c->points.p, c->h_stage, pbytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->eval_in.p, in4, sizeof(double) * 4 * n, hipMemcpyHostToDevice, c->stream));
+  const int grid = (n + 255) / 256;
+  if (f64)
+    eval_kernel<double><<<grid, 256, 0, c->stream>>>(op, (const Point<double>*)c->points.p,
+                                                     (const double*)c->eval_in.p, n, (double*)c->eval_out.p);
+  else
+    eval_kernel<float><<<grid, 256, 0, c->stream>>>(op, (const Point<float>*)c->points.p,
+                                                    (const double*)c->eval_in.p, n, (double*)c->eval_out.p);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipMemcpyAsync(out2, c->eval_out.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,207 @@
+"""GPU transport parity (MI355X): the HIP kernel through the C ABI against
+the oracle and against the reference's own golden fixtures."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import golden_cases, load_golden, spec_of, vec, zscores
+from ismc_amd import ISMC, ISMLMC, Layer, SampleConfig, Sphere
+from ismc_amd.workloads import single_layer, slab_between_slides, three_layer_sample
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    yield "slides", slab_between_slides()
+    yield "three_layer", three_layer_sample()
+    cfg = single_layer(1.0, 100.0, g=0.9)
+    yield "single", cfg
+    cfg = slab_between_slides()
+    cfg.R = Sphere(False, angle=0.1)
+    cfg.T = Sphere(False, angle=0.05)
+    yield "no_spheres", cfg
+    cfg = slab_between_slides()
+    cfg.specular_included = True
+    yield "specular", cfg
+    yield "iso_glass_behind", SampleConfig(
+        layers=[Layer(1.33, 0.0, 0.05, 0.3, 20.0), Layer.glass(1.5, 0.2)],
+        R=slab_between_slides().R, T=slab_between_slides().T)
+
+
+CASES = list(_cases())
+
+
+def test_device_is_gfx950(engine):
+    info = engine.device_info()
+    assert info["arch"].startswith("gfx950"), info
+    assert info["warp_size"] == 64
+
+
+def test_eval_physics_matches_oracle(engine):
+    rng = np.random.default_rng(1)
+    pairs = [(1.0, 1.4), (1.4, 1.0), (1.52, 1.4), (1.4, 1.52), (1.0, 1.52), (1.37, 1.44)]
+    rows = []
+    for ni, nt in pairs:
+        for mu in np.concatenate([rng.uniform(1e-4, 1, 200), [1.0, 0.999995, 0.5, -0.7]]):
+            rows.append((ni, nt, mu, 0.0))
+    x = np.array(rows)
+    ref = np.array([O.fresnel(a, b, c) for a, b, c, _ in rows])
+    for fp64, tol in ((True, 1e-12), (False, 3e-6)):
+        out = engine.eval_physics(0, x, fp64=fp64)
+        assert np.max(np.abs(out[:, 0] - ref)) < tol
+        ci = np.abs(x[:, 2])
+        eta = x[:, 0] / x[:, 1]
+        ct = np.sqrt(np.maximum(0, 1 - eta ** 2 * (1 - ci ** 2)))
+        assert np.max(np.abs(out[:, 1] - ct)) < (1e-12 if fp64 else 3e-6)
+    # Henyey-Greenstein inverse CDF
+    hg = np.array([(g, u, 0, 0) for g in (0.0, 0.5, 0.8, 0.9, 0.95) for u in rng.uniform(1e-6, 1, 100)])
+    refh = np.array([O.hg_cos(g, u) for g, u, _, _ in hg])
+    assert np.max(np.abs(engine.eval_physics(1, hg, fp64=True)[:, 0] - refh)) < 1e-12
+    assert np.max(np.abs(engine.eval_physics(1, hg)[:, 0] - refh)) < 5e-5
+    # cos(2 pi u) and -ln(u) at 24-bit grid points
+    u = (np.arange(1, 4097) * 4096.0) / 2 ** 24
+    uu = np.stack([u, 0 * u, 0 * u, 0 * u], 1)
+    assert np.max(np.abs(engine.eval_physics(2, uu)[:, 0] - np.cos(2 * np.pi * u))) < 2e-6
+    assert np.max(np.abs(engine.eval_physics(3, uu)[:, 0] + np.log(u)) / (1 - np.log(u))) < 2e-6
+    # glass entrance reflectance
+    cfg = slab_between_slides()
+    mus = np.concatenate([rng.uniform(0.01, 1, 100), -rng.uniform(0.01, 1, 100)])
+    refe = np.array([O.entrance_reflectance(spec_of(cfg), m) for m in mus])
+    e = engine.eval_physics(4, np.stack([mus, 0 * mus, 0 * mus, 0 * mus], 1), cfg=cfg, fp64=True)
+    assert np.max(np.abs(e[:, 0] - refe)) < 1e-12
+    # spin: mu_z' = mu_z cos_t - sqrt((1-cos_t^2)(1-mu_z^2)) cos(2 pi u_phi)
+    sp = np.array([(m, g, a, b) for m in (-1.0, -0.3, 0.0, 0.6, 0.999999) for g in (0.0, 0.9)
+                   for a, b in rng.uniform(1e-6, 1, (20, 2))])
+    out = engine.eval_physics(5, sp, fp64=True)
+    ct = out[:, 1]
+    gen = sp[:, 0] * ct - np.sqrt((1 - ct ** 2) * (1 - sp[:, 0] ** 2)) * np.cos(2 * np.pi * sp[:, 3])
+    expect = np.where(np.abs(sp[:, 0]) > 0.99999, np.sign(sp[:, 0]) * ct, gen)
+    assert np.max(np.abs(out[:, 0] - expect)) < 1e-12
+
+
+@pytest.mark.parametrize("name,cfg", CASES, ids=[c[0] for c in CASES])
+def test_fp64_kernel_matches_oracle_per_packet(engine, name, cfg):
+    n = 4000
+    _, g = engine.simulate_packets(cfg, n, seed=77, first_packet_id=1000, stream=3, fp64=True)
+    o = O.simulate(spec_of(cfg), n, seed=77, first_pid=1000, stream=3, per_packet=True)["per_packet"]
+    same = np.all(np.abs(g[:, :5] - o[:, :5]) < 1e-6, axis=1) & (g[:, 5] == o[:, 5])
+    print(name, "identical packets: %.5f" % same.mean())
+    assert same.mean() >= 0.999
+    assert np.abs(g[:, :5].sum(0) - o[:, :5].sum(0)).max() < 1e-3 * n ** 0.5
+
+
+@pytest.mark.parametrize("name,cfg", CASES, ids=[c[0] for c in CASES])
+def test_fp32_kernel_tracks_oracle(engine, name, cfg):
+    n = 4000
+    _, g = engine.simulate_packets(cfg, n, seed=5, fp64=False)
+    o = O.simulate(spec_of(cfg), n, seed=5, per_packet=True, flags=O.FP32)["per_packet"]
+    same = np.all(np.abs(g[:, :5] - o[:, :5]) < 1e-4, axis=1)
+    print(name, "FP32 packets within 1e-4 of the FP32 oracle: %.4f" % same.mean())
+    assert same.mean() >= 0.95
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_gpu_matches_reference_fixture(engine, case):
+    d = load_golden(case)
+    cfg = SampleConfig.from_spec(d["spec"])
+    n = 40 * d["n_per_batch"]
+    r = engine.simulate(cfg, n, seed=2024)
+    z = zscores(vec(r.mean), vec(r.se), d["mean"], d["se"])
+    print(case, np.round(z, 2))
+    assert r.n_aborted == 0
+    assert np.all(np.abs(z) < 3.0), dict(zip(("A", "R_d", "R_s", "T_d", "T_u"), np.round(z, 2)))
+
+
+def test_bitwise_reproducible_across_launch_shapes(engine):
+    cfg = slab_between_slides()
+    a = engine.simulate(cfg, 3_000_000, seed=9)
+    b = engine.simulate(cfg, 3_000_000, seed=9)
+    c = engine.simulate_batch([cfg], [3_000_000], seed=9, streams=[0], blocks_per_cu=2,
+                              chunk_packets=777)[0]
+    for k in a.sum:
+        assert a.sum[k] == b.sum[k] == c.sum[k]
+        assert a.sumsq[k] == b.sumsq[k] == c.sumsq[k]
+    assert a.events == b.events == c.events
+
+
+def test_batch_equals_individual_runs(engine):
+    cfgs = [slab_between_slides(), three_layer_sample(), single_layer(0.1, 30.0),
+            single_layer(5.0, 500.0, g=0.7)]
+    npk = [200_000, 50_000, 123_457, 31]
+    first = [0, 10, 5_000_000, 7]
+    streams = [4, 4, 9, 1]
+    batch = engine.simulate_batch(cfgs, npk, seed=31, first_packet_ids=first, streams=streams)
+    for cfg, n, f, s, b in zip(cfgs, npk, first, streams, batch):
+        one = engine.simulate(cfg, n, seed=31, first_packet_id=f, stream=s)
+        assert one.sum == b.sum and one.events == b.events and b.n_packets == n
+
+
+def test_large_run_properties(engine):
+    # 2^27 packets in one launch: energy balance and agreement with the oracle
+    cfg = slab_between_slides()
+    r = engine.simulate(cfg, 1 << 27, seed=1234)
+    assert r.n_aborted == 0
+    assert abs(sum(r.mean.values()) - 1.0) < 2e-4
+    o = O.simulate(spec_of(cfg), 1_000_000, seed=4321, threads=16)
+    z = zscores(vec(r.mean), vec(r.se), vec(o["mean"]), vec(o["se"]))
+    assert np.all(np.abs(z) < 3.0), np.round(z, 2)
+    ev = r.events
+    assert ev["entrances"] == ev["sphere_decisions"] and ev["reinjections"] < ev["entrances"]
+
+
+def test_fp32_vs_fp64_kernel_deviation(engine):
+    cfg = slab_between_slides()
+    n = 4_000_000
+    f32 = engine.simulate(cfg, n, seed=55)
+    f64 = engine.simulate(cfg, n, seed=55, fp64=True)
+    for k in f32.mean:
+        dev = f32.mean[k] - f64.mean[k]
+        print("FP32-FP64 %-3s %+.3e (SE %.3e)" % (k, dev, f64.se[k]))
+        assert abs(dev) < 3 * f64.se[k] + 1e-9
+
+
+def test_cuda_roulette_compat_only_loses_weight(engine):
+    cfg = slab_between_slides()
+    r = engine.simulate(cfg, 1_000_000, seed=3, cuda_roulette=True)
+    tot = sum(r.mean.values())
+    assert 1 - 1e-4 <= tot <= 1 + 1e-9
+
+
+def test_drop_in_propagators_match_fixtures(engine):
+    # ISMC.py's own example (ISMC.py:19-50), through the reference signature
+    d = load_golden("ismc_example")
+    L = d["spec"]["layers"][0]
+    R, T = d["spec"]["R"], d["spec"]["T"]
+    args = (400_000, L["n"], L["g"], L["t"], L["mu_a"], L["mu_s"], L["mu_a"] + L["mu_s"],
+            True, R["pw"], R["fs"], R["fp"], R["fd"], R["fs"] + R["fp"] + R["fd"], R["angle"], False,
+            False, T["pw"], T["fs"], T["fp"], T["fd"], T["fs"] + T["fp"] + T["fd"], T["angle"])
+    res = ISMC(*args, seed=12, engine=engine, return_result=True)
+    z = zscores(vec(res.mean), vec(res.se), d["mean"], d["se"])
+    assert np.all(np.abs(z) < 3.0), np.round(z, 2)
+    out = ISMC(*args, seed=12, engine=engine)
+    assert len(out) == 5 and abs(sum(out) - 1) < 0.01
+    # ISMLMC with glass slides
+    d = load_golden("cfg0_slides_dual")
+    ls = d["spec"]["layers"]
+    S = d["spec"]["R"]
+    col = lambda k: [x[k] for x in ls]
+    f = S["fs"] + S["fp"] + S["fd"]
+    res = ISMLMC(2_000_000, col("n"), col("g"), col("t"), col("mu_a"), col("mu_s"),
+                 [x["mu_a"] + x["mu_s"] for x in ls], True, S["pw"], S["fs"], S["fp"], S["fd"], f,
+                 0.0, False, True, S["pw"], S["fs"], S["fp"], S["fd"], f, 0.0, seed=13,
+                 engine=engine, return_result=True)
+    z = zscores(vec(res.mean), vec(res.se), d["mean"], d["se"])
+    assert np.all(np.abs(z) < 3.0), np.round(z, 2)
+
+
+def test_errors_are_raised_not_exited(engine):
+    bad = slab_between_slides()
+    bad.layers[1].g = 1.5
+    with pytest.raises(ValueError):
+        engine.simulate(bad, 1000)
+    with pytest.raises(ValueError):
+        engine.simulate(slab_between_slides(), (1 << 32) + 1)
+    ok = engine.simulate(slab_between_slides(), 1000)
+    assert ok.n_packets == 1000

@@ -1,0 +1,90 @@
+"""Oracle transport against closed forms, a literature known answer and
+invariants (CPU).  These pin the composite propagator (MLMC.py:76-202,
+ISMC.py:87-160, ISMLMC.py:101-347) where the mathematics fixes the answer."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+NO_SPHERE = {"present": 0, "angle": 0.0}
+
+
+def slab(n, mu_a, mu_s, g, t, **kw):
+    d = {"layers": [{"n": n, "g": g, "t": t, "mu_a": mu_a, "mu_s": mu_s}],
+         "R": dict(NO_SPHERE), "T": dict(NO_SPHERE), "specular_included": 0}
+    d.update(kw)
+    return d
+
+
+@pytest.mark.parametrize("flags", [0, O.MIRROR])
+def test_absorbing_slab_closed_form(flags):
+    # mu_s -> 0: every packet is absorbed whole at its first interaction, so
+    # only the ballistic multiple-reflection series leaves the slab.
+    n, mu_a, t = 1.5, 5.0, 0.1
+    r = ((n - 1) / (n + 1)) ** 2
+    E = math.exp(-mu_a * t)
+    Tu = (1 - r) ** 2 * E / (1 - r * r * E * E)
+    Rs = r + (1 - r) ** 2 * r * E * E / (1 - r * r * E * E)
+    N = 200000
+    res = O.simulate(slab(n, mu_a, 1e-9, 0.0, t), N, seed=11, flags=flags, threads=4)
+    for key, exp in (("T_u", Tu), ("R_s", Rs), ("A", 1 - Tu - Rs)):
+        se = math.sqrt(exp * (1 - exp) / N)
+        assert abs(res["mean"][key] - exp) < 3 * se + 1e-6, (key, res["mean"][key], exp)
+    assert res["mean"]["R_d"] < 1e-6 and res["mean"]["T_d"] < 1e-6
+
+
+def test_matched_boundary_known_answer():
+    # van de Hulst's tabulated case (n=1, albedo 0.9, optical thickness 2,
+    # g=0.75), the classic MCML check, here with the reference's sample
+    # definition of MC.py:22-26: mu_a=10, mu_s'=22.5 -> mu_s=90, t=0.02 cm.
+    N = 1000000
+    res = O.simulate(slab(1.0, 10.0, 90.0, 0.75, 0.02), N, seed=5, threads=8)
+    m = res["mean"]
+    # unscattered light is exactly the ballistic fraction exp(-mu_t t)
+    Tb = math.exp(-2.0)
+    assert abs(m["T_u"] - Tb) < 3 * math.sqrt(Tb * (1 - Tb) / N)
+    Rt = m["R_d"] + m["R_s"]
+    Tt = m["T_d"] + m["T_u"]
+    # published: R_d = 0.09739, T_t = 0.66096 (van de Hulst 1980 / MCML)
+    assert Rt == pytest.approx(0.09739, abs=0.0008)
+    assert Tt == pytest.approx(0.66096, abs=0.0012)
+
+
+def test_energy_conservation_per_packet():
+    spec = {"layers": [{"n": 1.52, "g": 0, "t": .1, "mu_a": 0, "mu_s": 0},
+                       {"n": 1.40, "g": .9, "t": .1, "mu_a": 1, "mu_s": 100},
+                       {"n": 1.52, "g": 0, "t": .1, "mu_a": 0, "mu_s": 0}],
+            "R": {"present": 1, "pw": .98, "fs": .0039, "fp": .0039, "fd": .001},
+            "T": {"present": 1, "pw": .98, "fs": .0039, "fp": .0039, "fd": .001}}
+    res = O.simulate(spec, 20000, seed=3, per_packet=True)
+    tot = res["per_packet"][:, :5].sum(axis=1)
+    # roulette keeps the expectation: mean 1 within its standard error
+    assert abs(tot.mean() - 1) < 4 * tot.std() / math.sqrt(tot.size) + 1e-12
+    # packets that were never rouletted carry exactly weight 1
+    assert np.mean(np.abs(tot - 1) < 1e-9) > 0.5
+    # CUDA-compat roulette (CUDAISMLMC.h:275, integer 1/m) only ever loses weight
+    rc = O.simulate(spec, 20000, seed=3, per_packet=True, flags=O.CUDA_ROULETTE)
+    tc = rc["per_packet"][:, :5].sum(axis=1)
+    assert np.all(tc <= 1 + 1e-12) and np.all(tc >= 1 - 2e-4)
+
+
+def test_mirror_equals_substep_per_packet():
+    # ISMC.py's mirror boundary and MLMC.py's substep boundary are the same
+    # geometry for one slab: with the same draws the histories coincide.
+    spec = slab(1.4, 1.0, 100.0, 0.9, 0.1, roulette_threshold=1e-4,
+                R={"present": 1, "pw": .98, "fs": .0039, "fp": .0039, "fd": .001},
+                T={"present": 1, "pw": .98, "fs": .0039, "fp": .0039, "fd": .001})
+    a = O.simulate(spec, 5000, seed=8, per_packet=True)["per_packet"]
+    b = O.simulate(spec, 5000, seed=8, per_packet=True, flags=O.MIRROR)["per_packet"]
+    same = np.all(np.abs(a - b) < 1e-9, axis=1)
+    assert same.mean() > 0.999
+
+
+def test_fp32_oracle_tracks_fp64():
+    spec = slab(1.4, 1.0, 100.0, 0.9, 0.1)
+    a = O.simulate(spec, 5000, seed=8, per_packet=True)["per_packet"]
+    b = O.simulate(spec, 5000, seed=8, per_packet=True, flags=O.FP32)["per_packet"]
+    same = np.all(np.abs(a[:, :5] - b[:, :5]) < 1e-4, axis=1)
+    assert same.mean() > 0.97
