@@ -62,6 +62,19 @@ constexpr uint32_t kMaxEventsPerPacket = 1u << 22;
 constexpr int kSlotWords = 16;  // per point: sum[5], sumsq[5], events[5], aborted
 
 enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2 };
+
+// A packet that refracts into a glass slide from a less dense medium a at
+// grazing incidence lands on the critical angle of the glass->a face.  In
+// finite precision 1 - mu^2 can round so that the way back reads as total
+// internal reflection; with TIR at the far face too (glass->air) the packet
+// would bounce in the non-absorbing slide for ever (the FP32 reference path
+// has no exit from that loop).  The refracted direction is therefore kept
+// at least kGlassGuard (relative, in sin^2) inside the critical cone of the
+// face it came through; this moves only packets within ~sqrt(kGlassGuard)
+// of grazing incidence.
+template <typename R> struct GlassGuard;
+template <> struct GlassGuard<float> { static constexpr float value = 9.5367431640625e-07f; };  // 2^-20
+template <> struct GlassGuard<double> { static constexpr double value = 9.094947017729282e-13; };  // 2^-40
 enum : int32_t { PH_NEED = 0, PH_ENTER = 1, PH_FLIGHT = 2, PH_DONE = 3 };
 
 // Constants of one layer, derived on the host in FP64.
@@ -209,6 +222,15 @@ __device__ __forceinline__ R spin_z(R mu_z, R cos_t, R u_phi) {
   const R general = mu_z * cos_t - st_sz * cphi;
   // near-vertical: mu_z' = sign(mu_z) * cos_t (cos_t keeps its own sign)
   return DM<R>::abs(mu_z) > (R)0.99999 ? (mu_z > (R)0 ? cos_t : -cos_t) : general;
+}
+
+// cosine inside a glass slide of index ng, entered from index na, clamped
+// away from the critical angle of the way back (see GlassGuard)
+template <typename R>
+__device__ __forceinline__ R glass_entry_cos(R ct, R na, R ng) {
+  const R q = na * DM<R>::rcp(ng);
+  const R cmin = DM<R>::sqrt(fmax((R)0, (R)1 - ((R)1 - GlassGuard<R>::value) * q * q));
+  return ng > na ? fmax(ct, cmin) : ct;
 }
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
@@ -384,6 +406,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
             to_sphere = true;
           } else {
             const Layer<R>& NL = P.layer[nxt];
+            if (NL.kind == KIND_GLASS) muz = DM<R>::csign(glass_entry_cos<R>(ct, L_n, NL.n), muz);
             if (NL.kind != KIND_GLASS && s != (R)0) s *= L_mut * DM<R>::rcp(NL.mu_t);
             layer = nxt;
             L_zlo = NL.zlo; L_zhi = NL.zhi; L_n = NL.n; L_step = NL.step_k; L_abs = NL.absfrac;
@@ -406,6 +429,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
         L_zlo = NL.zlo; L_zhi = NL.zhi; L_n = NL.n; L_step = NL.step_k; L_abs = NL.absfrac;
         L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
         L_mut = NL.mu_t; L_kind = NL.kind;
+        if (L_kind == KIND_GLASS) muz = DM<R>::csign(glass_entry_cos<R>(ct, P.n_amb, L_n), muz);
         z = side ? P.t_tot : (R)0;
         s = L_kind == KIND_GLASS ? (R)0 : DM<R>::nlog2(u1) * L_step;
         phase = PH_FLIGHT;

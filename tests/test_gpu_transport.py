@@ -48,13 +48,16 @@ def test_eval_physics_matches_oracle(engine):
             rows.append((ni, nt, mu, 0.0))
     x = np.array(rows)
     ref = np.array([O.fresnel(a, b, c) for a, b, c, _ in rows])
-    for fp64, tol in ((True, 1e-12), (False, 3e-6)):
+    ci = np.abs(x[:, 2])
+    eta = x[:, 0] / x[:, 1]
+    ct = np.sqrt(np.maximum(0, 1 - eta ** 2 * (1 - ci ** 2)))
+    # FP32: 1 - sin^2(theta_t) cancels near total internal reflection, so the
+    # error grows like eps / cos(theta_t); away from it the bound is flat.
+    tol32 = 5e-6 + 2e-6 / np.maximum(ct, 1e-4)
+    for fp64, tol in ((True, 1e-12), (False, tol32)):
         out = engine.eval_physics(0, x, fp64=fp64)
-        assert np.max(np.abs(out[:, 0] - ref)) < tol
-        ci = np.abs(x[:, 2])
-        eta = x[:, 0] / x[:, 1]
-        ct = np.sqrt(np.maximum(0, 1 - eta ** 2 * (1 - ci ** 2)))
-        assert np.max(np.abs(out[:, 1] - ct)) < (1e-12 if fp64 else 3e-6)
+        assert np.all(np.abs(out[:, 0] - ref) < tol)
+        assert np.all(np.abs(out[:, 1] - ct) < tol)
     # Henyey-Greenstein inverse CDF
     hg = np.array([(g, u, 0, 0) for g in (0.0, 0.5, 0.8, 0.9, 0.95) for u in rng.uniform(1e-6, 1, 100)])
     refh = np.array([O.hg_cos(g, u) for g, u, _, _ in hg])

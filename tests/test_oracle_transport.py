@@ -88,3 +88,20 @@ def test_fp32_oracle_tracks_fp64():
     b = O.simulate(spec, 5000, seed=8, per_packet=True, flags=O.FP32)["per_packet"]
     same = np.all(np.abs(a[:, :5] - b[:, :5]) < 1e-4, axis=1)
     assert same.mean() > 0.97
+
+
+def test_grazing_glass_entry_cannot_trap_a_packet():
+    # Packet 44473 of seed 2024 in FP32 reaches the lower slide at grazing
+    # incidence from the sample; without the glass-entry guard it bounces
+    # between total internal reflection at the air face and a rounding-made
+    # TIR at the sample face until the event cap stops it.
+    spec = {"layers": [{"n": 1.52, "g": 0, "t": .1, "mu_a": 0, "mu_s": 0},
+                       {"n": 1.40, "g": .9, "t": .1, "mu_a": 1, "mu_s": 100},
+                       {"n": 1.52, "g": 0, "t": .1, "mu_a": 0, "mu_s": 0}],
+            "R": {"present": 1, "pw": .98, "fs": 0.003921629, "fp": 0.003921629, "fd": 0.000977518},
+            "T": {"present": 1, "pw": .98, "fs": 0.003921629, "fp": 0.003921629, "fd": 0.000977518}}
+    r = O.simulate(spec, 1, seed=2024, first_pid=44473, flags=O.FP32, per_packet=True)
+    assert r["n_aborted"] == 0 and r["per_packet"][0, 5] < 100000
+    assert abs(r["per_packet"][0, :5].sum() - 1) < 1e-6 or r["per_packet"][0, :5].sum() < 1e-3
+    big = O.simulate(spec, 2_000_000, seed=2024, flags=O.FP32, threads=8)
+    assert big["n_aborted"] == 0
