@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run a plain HBM copy of this many bytes (byte-counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
@@ -97,6 +99,11 @@ def main():
     value = N * a.steps / elapsed
     launch = eng.last_launch()
 
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "transport_slab_between_slides_counters.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f)["hbm_bytes_per_packet"] * N
     W = slots_per_packet(events, N)
     avg_s = sum(kernel_ms) / len(kernel_ms) / 1e3
     achieved = W * N / avg_s / 1e12
@@ -120,7 +127,9 @@ def main():
                    "roulette": "1e-4, 1 in 10", "seed": 1234, "grid_blocks": launch["grid"],
                    "block_threads": launch["block"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tslot/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
+                     "traffic_note": "HBM bytes per launch: rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00 "
+                                     "(copy-calibrated) at 2^28 packets, per packet x N",
                      "slots_per_packet": W, "kernel_ms_avg": avg_s * 1e3,
                      "kernel_packets_per_s": N / avg_s,
                      "events_per_packet": {k: v / N for k, v in events.items()}},
@@ -132,6 +141,10 @@ def main():
         n64 = 1 << 26
         r64 = eng.simulate(cfg, n64, seed=78, fp64=True)
         out["fp64_kernel_packets_per_s"] = n64 / (r64.kernel_ms / 1e3)
+    if a.copy_probe:
+        ms = eng.copy_probe(a.copy_probe)
+        out["copy_probe"] = {"bytes": a.copy_probe, "kernel_ms": ms,
+                             "GB_per_s": 2 * a.copy_probe / ms / 1e6}
     if not a.no_cpu:
         print("bench: timing the CPU baseline (%.0f s)" % a.cpu_seconds, file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)

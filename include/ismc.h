@@ -147,6 +147,11 @@ int ismc_last_launch(ismc_ctx* ctx, float* kernel_ms, int32_t* grid_blocks,
 int ismc_eval_physics(ismc_ctx* ctx, int32_t op, const ismc_config* cfg, const double* in4,
                       int32_t n, uint32_t flags, double* out2);
 
+/* Copy `bytes` bytes between two device buffers with a plain streaming
+ * kernel and return its device time: a known-size reference for the
+ * hardware byte counters (FETCH_SIZE / WRITE_SIZE) of a profile run. */
+int ismc_copy_probe(ismc_ctx* ctx, int64_t bytes, float* kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

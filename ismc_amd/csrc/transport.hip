@@ -498,6 +498,11 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
   if (set_point1 != 0xffffffffu) flush_slot<R>(acc1, a.slots, set_point1, lane);
 }
 
+__global__ void copy_probe_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void zero_u64_kernel(unsigned long long* p, size_t n, uint32_t* counter) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -965,6 +970,32 @@ int ismc_eval_physics(ismc_ctx* c, int32_t op, const ismc_config* cfg, const dou
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipMemcpyAsync(out2, c->eval_out.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ismc_copy_probe(ismc_ctx* c, int64_t bytes, float* kernel_ms) {
+  if (!c || bytes <= 0 || (bytes % 16) != 0) return set_err(c, ISMC_EINVAL, "copy probe needs a positive multiple of 16 bytes%s");
+  HIPCHECK(c, hipSetDevice(c->device));
+  void* a = nullptr;
+  void* b = nullptr;
+  HIPCHECK(c, hipMalloc(&a, (size_t)bytes));
+  hipError_t e = hipMalloc(&b, (size_t)bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return set_err(c, ISMC_ENOMEM, "hipMalloc(%s) for the copy probe", hipGetErrorString(e));
+  }
+  const size_t n = (size_t)bytes / 16;
+  const int grid = c->prop.multiProcessorCount * 8;
+  copy_probe_kernel<<<grid, 256, 0, c->stream>>>((const uint4*)b, (uint4*)a, n);  // warm
+  HIPCHECK(c, hipEventRecord(c->e0, c->stream));
+  copy_probe_kernel<<<grid, 256, 0, c->stream>>>((const uint4*)a, (uint4*)b, n);
+  HIPCHECK(c, hipEventRecord(c->e1, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHECK(c, hipEventElapsedTime(&ms, c->e0, c->e1));
+  if (kernel_ms) *kernel_ms = ms;
+  (void)hipFree(a);
+  (void)hipFree(b);
   return 0;
 }
 
