@@ -140,6 +140,11 @@ int ismc_simulate_packets(ismc_ctx* ctx, const ismc_config* cfg, uint64_t n_pack
 int ismc_last_launch(ismc_ctx* ctx, float* kernel_ms, int32_t* grid_blocks,
                      int32_t* block_threads, int64_t* n_chunks);
 
+/* How many launch graphs were captured and how many graph replays ran
+ * (ismc_tune.use_graph = 1 replays a captured graph when a batch call
+ * repeats the shape, buffers and seed of an earlier one). */
+int ismc_graph_stats(ismc_ctx* ctx, int64_t* captures, int64_t* launches);
+
 /* Evaluate the device physics helpers on host inputs (parity aid):
  * op 0 fresnel(a,b,c)->out[0], cos_t(out[1]); op 1 hg_cos(g=a,u=b);
  * op 2 cos(2 pi a); op 3 -ln(a); op 4 entrance reflectance of cfg at mu_z=a

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libismc.so"
 
 EXPORTS = ("ismc_create", "ismc_destroy", "ismc_last_error", "ismc_device_info_get",
            "ismc_simulate", "ismc_simulate_batch", "ismc_simulate_packets", "ismc_last_launch",
-           "ismc_eval_physics", "ismc_validate_config", "ismc_copy_probe")
+           "ismc_eval_physics", "ismc_validate_config", "ismc_copy_probe", "ismc_graph_stats")
 
 
 class Sphere(ctypes.Structure):
@@ -101,6 +101,8 @@ def load():
         L.ismc_eval_physics.restype = ctypes.c_int
         L.ismc_validate_config.argtypes = [P(Config), ctypes.c_char_p, ctypes.c_int32]
         L.ismc_validate_config.restype = ctypes.c_int
+        L.ismc_graph_stats.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int64)]
+        L.ismc_graph_stats.restype = ctypes.c_int
         L.ismc_copy_probe.argtypes = [vp, ctypes.c_int64, P(ctypes.c_float)]
         L.ismc_copy_probe.restype = ctypes.c_int
         _lib = L

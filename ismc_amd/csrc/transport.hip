@@ -572,7 +572,24 @@ struct ismc_ctx {
   int64_t last_chunks = 0;
   int blocks_per_cu = 8;
   int chunk_packets = 4096;
+  // captured launch sequences, replayed when a call repeats a shape
+  struct Graph {
+    std::vector<uint64_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Graph> graphs;
+  int64_t graph_launches = 0;
+  int64_t graph_captures = 0;
 };
+
+static void drop_graphs(ismc_ctx* c) {
+  for (auto& g : c->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  c->graphs.clear();
+}
 
 static int set_err(ismc_ctx* c, int code, const char* fmt, const char* a = "", long long b = 0) {
   char buf[512];
@@ -588,8 +605,11 @@ static int set_err(ismc_ctx* c, int code, const char* fmt, const char* a = "", l
       return set_err(ctx, ISMC_EDEVICE, "%s (line %lld)", hipGetErrorString(e_), __LINE__); \
   } while (0)
 
+static void drop_graphs(ismc_ctx* c);
+
 static int ensure_dev(ismc_ctx* c, DevBuf& b, size_t bytes) {
   if (b.cap >= bytes && b.p) return 0;
+  drop_graphs(c);  // captured graphs hold the old addresses
   if (b.p) {
     hipError_t e = hipFree(b.p);
     (void)e;
@@ -609,6 +629,7 @@ static int ensure_dev(ismc_ctx* c, DevBuf& b, size_t bytes) {
 
 static int ensure_stage(ismc_ctx* c, size_t bytes) {
   if (c->h_stage_cap >= bytes) return 0;
+  drop_graphs(c);
   if (c->h_stage) {
     hipError_t e = hipHostFree(c->h_stage);
     (void)e;
@@ -762,14 +783,7 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   Point<R>* hp = reinterpret_cast<Point<R>*>(st);
   for (int p = 0; p < n_points; ++p) compile_point<R>(&cfgs[p], flags, &hp[p]);
   memcpy(st + pbytes, chunks.data(), sizeof(Chunk) * chunks.size());
-  HIPCHECK(c, hipMemcpyAsync(c->points.p, st, pbytes, hipMemcpyHostToDevice, c->stream));
-  if (!chunks.empty())
-    HIPCHECK(c, hipMemcpyAsync(c->chunks.p, st + pbytes, sizeof(Chunk) * chunks.size(),
-                               hipMemcpyHostToDevice, c->stream));
   const size_t nslot = (size_t)kSlotWords * n_points;
-  zero_u64_kernel<<<(unsigned)((nslot + 255) / 256), 256, 0, c->stream>>>(
-      (unsigned long long*)c->slots.p, nslot, (uint32_t*)c->counter.p);
-  HIPCHECK(c, hipGetLastError());
   if (per_packet_host) {
     if ((rc = ensure_dev(c, c->per_packet, sizeof(double) * 6 * std::max<uint64_t>(total, 1))))
       return rc;
@@ -797,15 +811,67 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   const uint64_t blocks_useful = (waves_useful + kWavesPerBlock - 1) / kWavesPerBlock;
   const int grid = (int)std::max<uint64_t>(
       1, std::min<uint64_t>((uint64_t)c->prop.multiProcessorCount * per_cu, blocks_useful));
-  HIPCHECK(c, hipEventRecord(c->e0, c->stream));
-  if (per_packet_host)
-    transport_kernel<R, true><<<grid, kBlock, 0, c->stream>>>(a);
-  else
-    transport_kernel<R, false><<<grid, kBlock, 0, c->stream>>>(a);
-  HIPCHECK(c, hipGetLastError());
-  HIPCHECK(c, hipEventRecord(c->e1, c->stream));
   unsigned long long* hs = reinterpret_cast<unsigned long long*>(st + pbytes + cbytes);
-  HIPCHECK(c, hipMemcpyAsync(hs, c->slots.p, sbytes, hipMemcpyDeviceToHost, c->stream));
+  const size_t cbytes_used = sizeof(Chunk) * chunks.size();
+  // the launch sequence: upload point tables and the chunk queue from pinned
+  // staging, clear the tally slots and queue counter with a kernel (no
+  // memset nodes), run the transport, download the slots
+  auto enqueue = [&](bool timed) -> int {
+    HIPCHECK(c, hipMemcpyAsync(c->points.p, st, pbytes, hipMemcpyHostToDevice, c->stream));
+    if (cbytes_used)
+      HIPCHECK(c, hipMemcpyAsync(c->chunks.p, st + pbytes, cbytes_used, hipMemcpyHostToDevice,
+                                 c->stream));
+    zero_u64_kernel<<<(unsigned)((nslot + 255) / 256), 256, 0, c->stream>>>(
+        (unsigned long long*)c->slots.p, nslot, (uint32_t*)c->counter.p);
+    HIPCHECK(c, hipGetLastError());
+    if (timed) HIPCHECK(c, hipEventRecord(c->e0, c->stream));
+    if (per_packet_host)
+      transport_kernel<R, true><<<grid, kBlock, 0, c->stream>>>(a);
+    else
+      transport_kernel<R, false><<<grid, kBlock, 0, c->stream>>>(a);
+    HIPCHECK(c, hipGetLastError());
+    if (timed) HIPCHECK(c, hipEventRecord(c->e1, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(hs, c->slots.p, sbytes, hipMemcpyDeviceToHost, c->stream));
+    return 0;
+  };
+  const bool use_graph = tune && tune->use_graph && !per_packet_host;
+  if (use_graph) {
+    const std::vector<uint64_t> key = {
+        sizeof(R), (uint64_t)n_points, (uint64_t)chunks.size(), (uint64_t)grid,
+        (uint64_t)(uintptr_t)c->points.p, (uint64_t)(uintptr_t)c->chunks.p,
+        (uint64_t)(uintptr_t)c->slots.p, (uint64_t)(uintptr_t)c->counter.p,
+        (uint64_t)(uintptr_t)c->h_stage, pbytes, cbytes, sbytes, seed};
+    ismc_ctx::Graph* hit = nullptr;
+    for (auto& g : c->graphs)
+      if (g.key == key) hit = &g;
+    if (!hit) {
+      if (c->graphs.size() >= 8) drop_graphs(c);
+      ismc_ctx::Graph g;
+      g.key = key;
+      HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int erc = enqueue(false);
+      hipGraph_t graph = nullptr;
+      const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+      if (erc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return erc;
+      }
+      if (ee != hipSuccess)
+        return set_err(c, ISMC_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+      g.graph = graph;
+      HIPCHECK(c, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+      c->graphs.push_back(g);
+      hit = &c->graphs.back();
+      c->graph_captures++;
+    }
+    // the graph's own duration (uploads + clear + transport + download)
+    HIPCHECK(c, hipEventRecord(c->e0, c->stream));
+    HIPCHECK(c, hipGraphLaunch(hit->exec, c->stream));
+    HIPCHECK(c, hipEventRecord(c->e1, c->stream));
+    c->graph_launches++;
+  } else {
+    if ((rc = enqueue(true))) return rc;
+  }
   if (per_packet_host)
     HIPCHECK(c, hipMemcpyAsync(per_packet_host, c->per_packet.p, sizeof(double) * 6 * total,
                                hipMemcpyDeviceToHost, c->stream));
@@ -870,6 +936,7 @@ void ismc_destroy(ismc_ctx* c) {
   if (!c) return;
   hipError_t e = hipSetDevice(c->device);
   if (c->stream) e = hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   DevBuf* bufs[] = {&c->points, &c->chunks, &c->slots, &c->counter, &c->per_packet, &c->eval_in,
                     &c->eval_out};
   for (DevBuf* b : bufs)
@@ -924,6 +991,13 @@ int ismc_simulate_packets(ismc_ctx* c, const ismc_config* cfg, uint64_t n_packet
   if (n_packets > (1ull << 26)) return set_err(c, ISMC_ERANGE, "per-packet output limited to 2^26 packets%s");
   return run_dispatch(c, cfg, &n_packets, &first_packet_id, &stream, 1, seed, flags, nullptr, out,
                       per_packet);
+}
+
+int ismc_graph_stats(ismc_ctx* c, int64_t* captures, int64_t* launches) {
+  if (!c) return ISMC_EINVAL;
+  if (captures) *captures = c->graph_captures;
+  if (launches) *launches = c->graph_launches;
+  return 0;
 }
 
 int ismc_last_launch(ismc_ctx* c, float* kernel_ms, int32_t* grid_blocks, int32_t* block_threads,

@@ -128,8 +128,11 @@ class Engine:
     def simulate_batch(self, cfgs: Sequence[SampleConfig], n_packets, seed: int = 1234,
                        first_packet_ids=None, streams=None, fp64: bool = False,
                        cuda_roulette: bool = False, blocks_per_cu: int = 0,
-                       chunk_packets: int = 0):
-        """Many configurations, uneven packet counts, one persistent launch."""
+                       chunk_packets: int = 0, use_graph: bool = False):
+        """Many configurations, uneven packet counts, one persistent launch.
+        use_graph replays a captured hipGraph of the launch sequence when the
+        call repeats an earlier one's shape and seed (the uploads inside the
+        graph read the new configurations from pinned staging)."""
         npts = len(cfgs)
         if npts == 0:
             return []
@@ -141,7 +144,7 @@ class Engine:
         st = None
         if streams is not None:
             st = np.ascontiguousarray(np.broadcast_to(np.asarray(streams, dtype=np.uint32), (npts,)))
-        tune = _lib.Tune(int(blocks_per_cu), int(chunk_packets), 0, 0)
+        tune = _lib.Tune(int(blocks_per_cu), int(chunk_packets), int(bool(use_graph)), 0)
         outs = (_lib.Tally * npts)()
         P64 = ctypes.POINTER(ctypes.c_uint64)
         P32 = ctypes.POINTER(ctypes.c_uint32)
@@ -153,6 +156,12 @@ class Engine:
         _lib.check(self._ctx, rc, "ismc_simulate_batch")
         ms = self.last_launch()["kernel_ms"]
         return [_result(outs[i], ms) for i in range(npts)]
+
+    def graph_stats(self):
+        cap = ctypes.c_int64()
+        lau = ctypes.c_int64()
+        self._L.ismc_graph_stats(self._ctx, ctypes.byref(cap), ctypes.byref(lau))
+        return {"captures": cap.value, "launches": lau.value}
 
     def copy_probe(self, nbytes: int) -> float:
         """Device ms of a plain device-to-device copy of nbytes (counter calibration)."""

@@ -1,0 +1,85 @@
+"""Batched launches on the GPU: the lookup sweep driver, the adaptive
+(pilot-sized) sweep, and hipGraph replay of a repeated batch."""
+import os
+
+import numpy as np
+import pytest
+
+from ismc_amd import ini as ini_mod
+from ismc_amd import lookup
+from ismc_amd.config import Sphere, mus_from_reduced
+from ismc_amd.workloads import lookup_grid, single_layer, standard_sphere
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_sample_space_matches_single_runs(engine):
+    R, T = standard_sphere(), standard_sphere()
+    mua, mus = [0.1, 1.0], [10.0, 30.0, 90.0]
+    res = lookup.sample_space(1.4, 0.8, 0.1, mua, mus, R, T, N=20000, runs=2, seed=5, engine=engine)
+    assert res.R_d.shape == (2, 3) and res.runs_R_d.shape == (2, 2, 3)
+    # point (i, j) of run r is batch entry (r * 6 + i * 3 + j) with that stream id
+    for r in range(2):
+        for i, a in enumerate(mua):
+            for j, s in enumerate(mus):
+                k = r * 6 + i * 3 + j
+                one = engine.simulate(lookup._single(1.4, 0.8, 0.1, a, mus_from_reduced(s, 0.8), R, T, False, 1e-4),
+                                      20000, seed=5, stream=k)
+                assert res.runs_R_d[r, i, j] == one.mean["R_d"]
+                assert res.runs_T_d[r, i, j] == one.mean["T_d"]
+    assert np.all(res.R_d_unc >= 0) and np.all(res.R_d > 0)
+
+
+def test_separate_spheres_and_no_spheres(engine):
+    R, T = standard_sphere(), standard_sphere()
+    sep = lookup.sample_space(1.4, 0.8, 0.1, [1.0], [10.0], R, T, N=50000, runs=0, separate=True,
+                              seed=3, engine=engine)
+    # runs == 0: Poisson uncertainty sqrt(x / N)
+    assert np.allclose(sep.R_d_unc, np.sqrt(sep.R_d / 50000))
+    none = lookup.sample_space(1.4, 0.8, 0.1, [1.0], [10.0], Sphere(False, angle=0.0),
+                               Sphere(False, angle=0.0), N=50000, runs=1, seed=3, engine=engine)
+    # without spheres the table holds total R and T (ISMLMC-lookup.cu:299-306)
+    assert none.R_d[0, 0] > sep.R_d[0, 0] * 0.5
+    assert 0 < none.R_d[0, 0] + none.T_d[0, 0] <= 1
+
+
+def test_graph_replay_is_bitwise_identical(engine):
+    cfgs = [single_layer(a, s) for a in (0.1, 1.0, 5.0) for s in (10.0, 100.0)]
+    eager = engine.simulate_batch(cfgs, 30000, seed=17)
+    before = engine.graph_stats()
+    for rep in range(4):
+        g = engine.simulate_batch(cfgs, 30000, seed=17, use_graph=True)
+        assert [x.sum for x in g] == [x.sum for x in eager]
+        # unrelated eager work between replays
+        engine.simulate(single_layer(2.0, 50.0), 10000, seed=rep)
+    # a replay with new configurations of the same shape reads them from staging
+    cfgs2 = [single_layer(a * 2, s) for a in (0.1, 1.0, 5.0) for s in (10.0, 100.0)]
+    g2 = engine.simulate_batch(cfgs2, 30000, seed=17, use_graph=True)
+    e2 = engine.simulate_batch(cfgs2, 30000, seed=17)
+    assert [x.sum for x in g2] == [x.sum for x in e2]
+    after = engine.graph_stats()
+    assert after["captures"] - before["captures"] == 1
+    assert after["launches"] - before["launches"] == 5
+
+
+def test_adaptive_sweep_reaches_target_rse(engine):
+    mua, mus, cfgs = lookup_grid(4, 4)
+    out, info = lookup.adaptive_sweep(cfgs, rse=0.005, pilot=10000, n_min=10000, n_max=1_000_000,
+                                      seed=11, engine=engine)
+    for o in out:
+        assert o["rse_R"] <= 0.0056 or o["n_packets"] == 1_000_000
+        assert 10000 <= o["n_packets"] <= 1_000_000
+    assert info["total_packets"] == sum(o["n_packets"] for o in out)
+
+
+def test_run_lookup_from_ini(engine, tmp_path):
+    f = ini_mod.load(os.path.join(HERE, "golden", "sample_lookup.ini"))
+    out = lookup.run_lookup(f, seed=2, engine=engine, rng_seed=4)
+    assert out["R_d"].shape == (3, 4) and out["samples_R_d"].shape == (3, 3, 4)
+    assert np.all(out["R_d_std"] > 0)
+    res = lookup.sample_space(f.n, f.g, f.t, *f.lookup_grid(), f.R.sphere(), f.T.sphere(),
+                              N=20000, runs=2, engine=engine)
+    pr, pt = lookup.write_tables(res, str(tmp_path))
+    m, s = lookup.read_csv_table(pr, 3)
+    assert np.allclose(m, res.R_d, rtol=1e-5) and np.allclose(s, res.R_d_unc, rtol=1e-5, atol=1e-12)
