@@ -1,0 +1,65 @@
+"""Inverse solver on the GPU: PRAXIS over the Monte Carlo cost."""
+import os
+
+import numpy as np
+import pytest
+
+from ismc_amd import ini as ini_mod
+from ismc_amd import minimization as M
+from ismc_amd.config import Sphere
+from ismc_amd.workloads import standard_sphere
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def synthetic(engine, mu_a, mus_r, n=1_000_000, stream=999):
+    p = M.Problem(n=1.40, g=0.8, t=0.1, R=standard_sphere(), T=standard_sphere(), measured=M.Measured())
+    cfg = p.configs(mu_a, mus_r)[0][0]
+    r = engine.simulate(cfg, n, seed=4242, stream=stream)
+    p.measured = M.Measured(R_d=r.mean["R_d"], T_d=r.mean["T_d"])
+    return p
+
+
+def test_recovers_synthetic_truth(engine):
+    p = synthetic(engine, 1.5, 20.0)
+    r = M.solve(p, (1.0, 10.0), n_packets=200_000, step=5.0, tol=0.01, engine=engine)
+    print("recovered", r.mu_a, r.mus_reduced, "evals", r.evaluations)
+    assert r.mu_a == pytest.approx(1.5, rel=0.03)
+    assert r.mus_reduced == pytest.approx(20.0, rel=0.03)
+
+
+def test_batched_solves_equal_sequential_solves(engine):
+    probs = [synthetic(engine, a, s, n=200_000, stream=900 + k)
+             for k, (a, s) in enumerate([(0.5, 10.0), (2.0, 30.0), (1.0, 15.0)])]
+    res, stats = M.solve_batch(probs, [(1.0, 12.0)] * 3, n_packets=20_000, step=5.0, tol=0.05,
+                               runs=1, engine=engine, max_retries=0)
+    assert stats["launches"] < sum(r.evaluations for r in res)
+    for p, (k, r) in zip(probs, enumerate(res)):
+        x, fx, nf, _ = M.solve_sequential(p, (1.0, 12.0), n_packets=20_000, step=5.0, tol=0.05,
+                                          engine=engine, stream=k)
+        assert np.array_equal(x, [r.mu_a, r.mus_reduced]) and nf == r.evaluations
+
+
+def test_reference_example_measurement(engine):
+    # Minimization/example.ini: n=1.4, g=0.5, t=0.135 cm, single-sphere
+    # (separate) readings M_R=0.2716, M_T=0.0853, M_U=0.000022; the file's
+    # accept window mu_a in [1, 3], mu_s' in [20, 40] brackets the answer.
+    R = Sphere(True, 0.99, 0.023510, 0.054638, 0.023510)
+    T = Sphere(True, 0.99, 0.054638, 0.023510, 0.023510, angle=np.radians(0.219523))
+    p = M.Problem(n=1.4, g=0.5, t=0.135, R=R, T=T, separate=True,
+                  measured=M.Measured(R_d=0.2716, T_d=0.0853, T_u=0.000022))
+    res, _ = M.solve_batch([p], [(5.0, 45.0)], n_packets=100_000, step=5.0, tol=0.1, runs=3,
+                           window=((1.0, 3.0), (20.0, 40.0)), engine=engine)
+    r = res[0]
+    print("example solution", r.stdout().split(), "retries", r.retries)
+    assert 1.0 <= r.mu_a <= 3.0 and 20.0 <= r.mus_reduced <= 40.0
+    assert r.retries < 5
+
+
+def test_run_minimization_from_ini(engine):
+    f = ini_mod.load(os.path.join(HERE, "golden", "sample_lookup.ini"))
+    out = M.run_minimization(f, samples=2, engine=engine, rng_seed=3)
+    assert np.isfinite(out["mu_a"]) and np.isfinite(out["mus_reduced"])
+    assert 0.01 <= out["mu_a"] <= 10 and 1 <= out["mus_reduced"] <= 100
+    assert out["stats"]["launches"] > 0
