@@ -157,6 +157,34 @@ int ismc_eval_physics(ismc_ctx* ctx, int32_t op, const ismc_config* cfg, const d
  * hardware byte counters (FETCH_SIZE / WRITE_SIZE) of a profile run. */
 int ismc_copy_probe(ismc_ctx* ctx, int64_t bytes, float* kernel_ms);
 
+/* ---------------------------------------------------------------------
+ * Surrogate MLP (no counterpart in the reference: it seeds the inverse
+ * solver).  n_in -> [hidden SiLU] x depth -> n_out, BF16 MFMA with FP32
+ * accumulation, FP32 master weights, Adam.  Parameters are exchanged in a
+ * flat logical layout: for each layer W[in][out] row-major, then b[out].
+ * Batches are multiples of 64 rows, at most max_batch.
+ * --------------------------------------------------------------------- */
+typedef struct ismc_mlp ismc_mlp;
+int ismc_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t depth, int32_t n_out,
+                    int32_t max_batch, ismc_mlp** out);
+void ismc_mlp_destroy(ismc_mlp* mlp);
+const char* ismc_mlp_last_error(const ismc_mlp* mlp);
+int64_t ismc_mlp_n_params(const ismc_mlp* mlp);
+int ismc_mlp_set_params(ismc_mlp* mlp, const float* flat);
+int ismc_mlp_get_params(ismc_mlp* mlp, float* flat);
+/* y[n][n_out] = f(x[n][n_in]) (any n; padded internally) */
+int ismc_mlp_forward(ismc_mlp* mlp, const float* x, int32_t n, float* y);
+/* gradient of the mean squared error over the batch, flat logical layout */
+int ismc_mlp_gradients(ismc_mlp* mlp, const float* x, const float* y, int32_t batch,
+                       float* grads, float* loss);
+/* keep a training set resident in device memory */
+int ismc_mlp_load_dataset(ismc_mlp* mlp, const float* x, const float* y, int64_t n);
+/* epochs x steps_per_epoch Adam steps on batches drawn in the order perm */
+int ismc_mlp_train(ismc_mlp* mlp, const int32_t* perm, int32_t epochs, int32_t steps_per_epoch,
+                   int32_t batch, float lr, float beta1, float beta2, float eps,
+                   float* epoch_loss, float* total_ms);
+int ismc_mlp_last_ms(ismc_mlp* mlp, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,260 @@
+"""Learned surrogate of the transport: an MLP mapping (log mu_a, log mu_s,
+g, n, thickness) to logit-transformed sphere readings (R_d, T_d, T_u and
+total reflectance R_d + R_s), trained only on tallies of this Monte Carlo.
+It seeds the inverse solver; MC re-simulation at the solution judges it.
+
+The network runs on libismc's BF16-MFMA kernels (ismc_mlp_* in
+include/ismc.h): 5 -> [256 SiLU] x 4 -> 4, FP32 master weights and Adam.
+"""
+import ctypes
+import os
+import time
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .config import Layer, SampleConfig
+from .engine import Engine, default_engine
+from .workloads import standard_sphere
+
+# input box (uniform in these coordinates): log10 mu_a, log10 mu_s, g, n, t
+RANGES = np.array([[-2.0, 1.0],       # mu_a 0.01 .. 10 /cm
+                   [0.0, 3.0],        # mu_s 1 .. 1000 /cm
+                   [0.60, 0.95],      # g
+                   [1.33, 1.50],      # n
+                   [0.05, 0.50]])     # t / cm
+OUTPUTS = ("R_d", "T_d", "T_u", "R_tot")
+CLAMP = 1e-6
+
+
+def features(mua, mus, g, n, t):
+    """Inputs scaled to [-1, 1] over the training box."""
+    raw = np.stack([np.log10(np.asarray(mua, dtype=np.float64)),
+                    np.log10(np.asarray(mus, dtype=np.float64)),
+                    np.asarray(g, dtype=np.float64) + 0 * np.asarray(mua, dtype=np.float64),
+                    np.asarray(n, dtype=np.float64) + 0 * np.asarray(mua, dtype=np.float64),
+                    np.asarray(t, dtype=np.float64) + 0 * np.asarray(mua, dtype=np.float64)], axis=-1)
+    lo, hi = RANGES[:, 0], RANGES[:, 1]
+    return (2.0 * (raw - lo) / (hi - lo) - 1.0).astype(np.float32)
+
+
+def logit(p):
+    p = np.clip(np.asarray(p, dtype=np.float64), CLAMP, 1 - CLAMP)
+    return np.log(p / (1 - p))
+
+
+def expit(z):
+    return 1.0 / (1.0 + np.exp(-np.asarray(z, dtype=np.float64)))
+
+
+def sample_inputs(n, seed=0):
+    """Uniform in (log mu_a, log mu_s, g, n, t) over the training box."""
+    rng = np.random.default_rng(seed)
+    u = rng.random((n, 5))
+    v = RANGES[:, 0] + u * (RANGES[:, 1] - RANGES[:, 0])
+    return {"mu_a": 10 ** v[:, 0], "mu_s": 10 ** v[:, 1], "g": v[:, 2], "n": v[:, 3], "t": v[:, 4]}
+
+
+def sample_config(mu_a, mu_s, g, n, t):
+    return SampleConfig(layers=[Layer(n=float(n), g=float(g), t=float(t), mu_a=float(mu_a),
+                                      mu_s=float(mu_s))],
+                        R=standard_sphere(), T=standard_sphere(), roulette_threshold=1e-4)
+
+
+def generate_dataset(n_samples, packets, seed=0, engine: Optional[Engine] = None,
+                     points_per_launch=16384, progress=None):
+    """MC labels for n_samples inputs: one batched launch per
+    points_per_launch samples (one work-queue chunk per sample).  Returns
+    (inputs dict, tallies [n, 4] in OUTPUTS order, seconds, packets)."""
+    eng = engine or default_engine()
+    inp = sample_inputs(n_samples, seed)
+    tal = np.zeros((n_samples, 4))
+    t0 = time.perf_counter()
+    for s in range(0, n_samples, points_per_launch):
+        e = min(n_samples, s + points_per_launch)
+        cfgs = [sample_config(inp["mu_a"][i], inp["mu_s"][i], inp["g"][i], inp["n"][i], inp["t"][i])
+                for i in range(s, e)]
+        res = eng.simulate_batch(cfgs, packets, seed=seed + 7, streams=np.arange(s, e, dtype=np.uint32),
+                                 chunk_packets=min(int(packets), 1 << 20))
+        for k, r in enumerate(res):
+            m = r.mean
+            tal[s + k] = (m["R_d"], m["T_d"], m["T_u"], m["R_d"] + m["R_s"])
+        if progress:
+            progress(e, n_samples, time.perf_counter() - t0)
+    return inp, tal, time.perf_counter() - t0, n_samples * int(packets)
+
+
+class MLP:
+    """Handle on one ismc_mlp (weights live in device memory)."""
+
+    def __init__(self, n_in=5, hidden=256, depth=4, n_out=4, max_batch=4096, device=0):
+        self._L = _lib.load()
+        L = self._L
+        L.ismc_mlp_create.argtypes = [ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+        L.ismc_mlp_create.restype = ctypes.c_int
+        L.ismc_mlp_destroy.argtypes = [ctypes.c_void_p]
+        L.ismc_mlp_destroy.restype = None
+        L.ismc_mlp_last_error.argtypes = [ctypes.c_void_p]
+        L.ismc_mlp_last_error.restype = ctypes.c_char_p
+        L.ismc_mlp_n_params.argtypes = [ctypes.c_void_p]
+        L.ismc_mlp_n_params.restype = ctypes.c_int64
+        for name in ("ismc_mlp_set_params", "ismc_mlp_get_params"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+            getattr(L, name).restype = ctypes.c_int
+        L.ismc_mlp_forward.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
+        L.ismc_mlp_forward.restype = ctypes.c_int
+        L.ismc_mlp_gradients.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.ismc_mlp_gradients.restype = ctypes.c_int
+        L.ismc_mlp_load_dataset.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+        L.ismc_mlp_load_dataset.restype = ctypes.c_int
+        L.ismc_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.ismc_mlp_train.restype = ctypes.c_int
+        L.ismc_mlp_last_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.ismc_mlp_last_ms.restype = ctypes.c_int
+        self._h = ctypes.c_void_p()
+        rc = L.ismc_mlp_create(device, n_in, hidden, depth, n_out, max_batch, ctypes.byref(self._h))
+        if rc != 0:
+            raise RuntimeError("ismc_mlp_create failed with status %d (needs a GPU)" % rc)
+        self.arch = (n_in, hidden, depth, n_out)
+        self.max_batch = max_batch
+        self.n_params = int(L.ismc_mlp_n_params(self._h))
+
+    def close(self):
+        if self._h:
+            self._L.ismc_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.ismc_mlp_last_error(self._h).decode(errors="replace")
+            raise RuntimeError("%s: %s (status %d)" % (what, msg, rc))
+
+    def set_params(self, flat):
+        f = np.ascontiguousarray(flat, dtype=np.float32)
+        assert f.size == self.n_params
+        self._check(self._L.ismc_mlp_set_params(self._h, f.ctypes.data), "set_params")
+
+    def get_params(self):
+        f = np.zeros(self.n_params, dtype=np.float32)
+        self._check(self._L.ismc_mlp_get_params(self._h, f.ctypes.data), "get_params")
+        return f
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.arch[3]), dtype=np.float32)
+        self._check(self._L.ismc_mlp_forward(self._h, x.ctypes.data, x.shape[0], y.ctypes.data), "forward")
+        return y
+
+    def last_ms(self):
+        ms = ctypes.c_float()
+        self._L.ismc_mlp_last_ms(self._h, ctypes.byref(ms))
+        return ms.value
+
+    def gradients(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(self._L.ismc_mlp_gradients(self._h, x.ctypes.data, y.ctypes.data, x.shape[0],
+                                               g.ctypes.data, ctypes.byref(loss)), "gradients")
+        return float(loss.value), g
+
+    def load_dataset(self, x, y):
+        self._x = np.ascontiguousarray(x, dtype=np.float32)
+        self._y = np.ascontiguousarray(y, dtype=np.float32)
+        self._check(self._L.ismc_mlp_load_dataset(self._h, self._x.ctypes.data, self._y.ctypes.data,
+                                                  self._x.shape[0]), "load_dataset")
+
+    def train(self, epochs, batch=4096, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=0, n_rows=None):
+        n = int(n_rows or self._x.shape[0])
+        steps = n // batch
+        rng = np.random.default_rng(seed)
+        perm = np.concatenate([rng.permutation(n)[:steps * batch] for _ in range(epochs)]).astype(np.int32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        ms = ctypes.c_float()
+        self._check(self._L.ismc_mlp_train(self._h, perm.ctypes.data, epochs, steps, batch, lr, beta1,
+                                           beta2, eps, losses.ctypes.data, ctypes.byref(ms)), "train")
+        return losses, ms.value
+
+
+@dataclass
+class Surrogate:
+    """Trained network plus the transform between tallies and outputs."""
+    mlp: MLP
+    train_info: dict
+
+    def predict(self, mua, mus, g, n, t):
+        x = features(mua, mus, g, n, t)
+        z = self.mlp.forward(x.reshape(-1, 5))
+        p = expit(z)
+        return {k: p[:, i] for i, k in enumerate(OUTPUTS)}
+
+    def save(self, path):
+        np.savez(path, params=self.mlp.get_params(), arch=np.array(self.mlp.arch),
+                 ranges=RANGES, info=np.array([repr(self.train_info)]))
+
+    @classmethod
+    def load(cls, path, device=0, max_batch=4096):
+        d = np.load(path, allow_pickle=False)
+        arch = tuple(int(v) for v in d["arch"])
+        mlp = MLP(*arch, max_batch=max_batch, device=device)
+        mlp.set_params(d["params"])
+        return cls(mlp, {"loaded_from": os.path.basename(path)})
+
+
+def init_params(arch, seed=0):
+    """He-style initialisation (N(0, 2/fan_in) for SiLU layers, 1/fan_in at
+    the output), flat logical layout."""
+    rng = np.random.default_rng(seed)
+    dims = [arch[0]] + [arch[1]] * arch[2] + [arch[3]]
+    parts = []
+    for i in range(len(dims) - 1):
+        k, n = dims[i], dims[i + 1]
+        scale = np.sqrt((2.0 if i < len(dims) - 2 else 1.0) / k)
+        parts += [rng.normal(0, scale, k * n), np.zeros(n)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def train_surrogate(inputs, tallies, epochs=60, batch=4096, lr=2e-3, seed=0, holdout=0.05,
+                    device=0, lr_decay_every=20):
+    """Fit the MLP to MC tallies (MSE on logits); returns (Surrogate,
+    held-out errors)."""
+    n = tallies.shape[0]
+    x = features(inputs["mu_a"], inputs["mu_s"], inputs["g"], inputs["n"], inputs["t"])
+    y = logit(tallies).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(n)
+    n_hold = max(batch, int(n * holdout))
+    hold, tr = order[:n_hold], order[n_hold:]
+    mlp = MLP(max_batch=batch, device=device)
+    mlp.set_params(init_params(mlp.arch, seed))
+    mlp.load_dataset(x[tr], y[tr])
+    losses, ms = [], 0.0
+    done = 0
+    while done < epochs:
+        e = min(lr_decay_every, epochs - done)
+        l, m = mlp.train(e, batch=batch, lr=lr * (0.3 ** (done // lr_decay_every)), seed=seed + done)
+        losses += list(map(float, l))
+        ms += m
+        done += e
+    pred = mlp.forward(x[hold])
+    err = {k: float(np.sqrt(np.mean((expit(pred[:, i]) - tallies[hold, i]) ** 2)))
+           for i, k in enumerate(OUTPUTS)}
+    rel = {k: float(np.median(np.abs(expit(pred[:, i]) - tallies[hold, i]) /
+                              np.maximum(tallies[hold, i], 1e-3))) for i, k in enumerate(OUTPUTS)}
+    info = {"epochs": epochs, "batch": batch, "lr": lr, "losses": losses, "train_ms": ms,
+            "n_train": int(tr.size), "n_holdout": int(hold.size), "holdout_rmse": err,
+            "holdout_median_rel": rel}
+    return Surrogate(mlp, info), info

@@ -1,0 +1,81 @@
+"""Surrogate MLP kernels (BF16 MFMA, FP32 accumulate) against the FP64
+oracle of the same network, and end-to-end training on MC labels."""
+import numpy as np
+import pytest
+
+from ismc_amd import surrogate as S
+from oracle import mlp_ref
+
+pytestmark = pytest.mark.gpu
+ARCH = (5, 256, 4, 4)
+
+
+@pytest.fixture(scope="module")
+def mlp():
+    m = S.MLP(*ARCH, max_batch=4096)
+    yield m
+    m.close()
+
+
+def test_param_roundtrip(mlp):
+    assert mlp.n_params == 199940 == mlp_ref.n_params(*ARCH)
+    w = S.init_params(ARCH, seed=3)
+    mlp.set_params(w)
+    assert np.array_equal(mlp.get_params(), w)
+
+
+def test_forward_matches_fp64_oracle(mlp):
+    w = S.init_params(ARCH, seed=5)
+    mlp.set_params(w)
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (1000, 5)).astype(np.float32)
+    y = mlp.forward(x)
+    yr, _ = mlp_ref.forward(w.astype(np.float64), x.astype(np.float64), ARCH)
+    err = np.abs(y - yr)
+    scale = np.abs(yr).mean()
+    print("forward max abs err %.3e (mean |y| %.3e)" % (err.max(), scale))
+    assert err.max() < 0.03 * scale + 1e-3
+
+
+def test_gradients_match_fp64_oracle(mlp):
+    w = S.init_params(ARCH, seed=7)
+    mlp.set_params(w)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, (4096, 5)).astype(np.float32)
+    t = rng.normal(0, 1, (4096, 4)).astype(np.float32)
+    loss, g = mlp.gradients(x, t)
+    lr, gr = mlp_ref.loss_and_grad(w.astype(np.float64), x.astype(np.float64), t.astype(np.float64), ARCH)
+    cos = float(np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr)))
+    rel = float(np.linalg.norm(g - gr) / np.linalg.norm(gr))
+    print("loss gpu %.6f oracle %.6f, grad cosine %.6f, relative L2 %.4f" % (loss, lr, cos, rel))
+    assert loss == pytest.approx(lr, rel=0.02)
+    assert cos > 0.999 and rel < 0.05
+    # per layer (weights and biases separately)
+    o = 0
+    for k, n in mlp_ref.shapes(*ARCH):
+        for size in (k * n, n):
+            a, b = g[o:o + size], gr[o:o + size]
+            if np.linalg.norm(b) > 0:
+                assert np.linalg.norm(a - b) / np.linalg.norm(b) < 0.08
+            o += size
+
+
+def test_training_fits_a_smooth_function(mlp):
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, (65536, 5)).astype(np.float32)
+    y = np.stack([np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2], np.tanh(x[:, 3] - x[:, 4]),
+                  x[:, 0] ** 2 - x[:, 1], np.cos(x[:, 2] + x[:, 4])], 1).astype(np.float32)
+    mlp.set_params(S.init_params(ARCH, seed=1))
+    mlp.load_dataset(x, y)
+    losses, ms = mlp.train(15, batch=4096, lr=2e-3, seed=0)
+    print("losses", np.round(losses, 5), "ms", ms)
+    assert losses[-1] < 0.05 * losses[0] and losses[-1] < 5e-3
+
+
+def test_surrogate_pipeline_small(engine):
+    inp, tal, secs, packets = S.generate_dataset(8192, 4000, seed=11, engine=engine)
+    assert tal.shape == (8192, 4) and np.all((tal >= 0) & (tal <= 1))
+    sur, info = S.train_surrogate(inp, tal, epochs=20, batch=1024, lr=2e-3, lr_decay_every=10)
+    print("holdout", info["holdout_rmse"], "train ms", info["train_ms"])
+    assert info["losses"][-1] < info["losses"][0]
+    assert info["holdout_rmse"]["R_d"] < 0.05 and info["holdout_rmse"]["T_d"] < 0.05
