@@ -1,0 +1,206 @@
+"""Batched inverse solving with a learned seed.
+
+Pipeline for many measurements (R_d, T_d of a single-layer sample with
+known g, n, t between the standard spheres):
+
+1. seed      the surrogate is evaluated once on a dense (mu_a, mu_s) grid at
+             the sample's (g, n, t); each measurement takes the grid point of
+             least relative misfit, polished by PRAXIS on the surrogate itself
+             (all measurements advance together, one network batch per round);
+2. refine    PRAXIS on the Monte Carlo cost in (ln mu_a, ln mu_s') from the
+             seed, every round of every solve in one transport launch, each
+             solve on its own Philox stream (common random numbers);
+3. judge     one independent MC run per solution at `check_packets`; a
+             solution whose R_d or T_d misses the measurement by more than
+             `miss_tol` (relative) is re-solved by plain MC minimization.
+
+The plain method (`solve_plain`) is the reference's: sequential PRAXIS in
+(mu_a, mu_s') with one MC launch per cost evaluation
+(CUDA/ISMLMC-minimization.cu:467-488).
+"""
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import minimization as M
+from . import praxis as _praxis
+from .config import mus_from_reduced
+from .engine import Engine, default_engine
+from .workloads import standard_sphere
+
+G, N_INDEX, THICK = 0.8, 1.40, 0.1
+
+
+def problem(R_d, T_d, g=G, n=N_INDEX, t=THICK):
+    return M.Problem(n=n, g=g, t=t, R=standard_sphere(), T=standard_sphere(),
+                     measured=M.Measured(R_d=float(R_d), T_d=float(T_d)))
+
+
+def make_measurements(count, packets=1_000_000, noise=0.01, seed=0, engine: Optional[Engine] = None,
+                      g=G, n=N_INDEX, t=THICK):
+    """Ground truths log-uniform in mu_a 0.01..10 /cm, mu_s 1..1000 /cm;
+    sphere readings simulated with `packets` photons, then multiplied by
+    (1 + noise * N(0, 1))."""
+    eng = engine or default_engine()
+    rng = np.random.default_rng(seed)
+    mua = 10 ** rng.uniform(-2, 1, count)
+    mus = 10 ** rng.uniform(0, 3, count)
+    musr = mus * (1 - g)
+    base = problem(0.5, 0.5, g, n, t)
+    cfgs = [base.configs(a, s)[0][0] for a, s in zip(mua, musr)]
+    res = eng.simulate_batch(cfgs, packets, seed=seed + 101, streams=np.arange(count, dtype=np.uint32) + 50_000_000)
+    Rd = np.array([r.mean["R_d"] for r in res])
+    Td = np.array([r.mean["T_d"] for r in res])
+    Rn = Rd * (1 + noise * rng.standard_normal(count))
+    Tn = Td * (1 + noise * rng.standard_normal(count))
+    return {"mu_a": mua, "mus_reduced": musr, "R_d_true": Rd, "T_d_true": Td, "R_d": Rn, "T_d": Tn,
+            "kernel_ms": res[0].kernel_ms}
+
+
+def surrogate_seeds(sur, R_d, T_d, g=G, n=N_INDEX, t=THICK, grid=256, polish=True):
+    """Initial (mu_a, mu_s') for each measurement from the surrogate."""
+    la = np.linspace(-2, 1, grid)
+    ls = np.linspace(0, 3, grid)
+    A, Sg = np.meshgrid(la, ls, indexing="ij")
+    pred = sur.predict(10 ** A.ravel(), 10 ** Sg.ravel(), g, n, t)
+    pR, pT = pred["R_d"], pred["T_d"]
+    R_d = np.asarray(R_d)
+    T_d = np.asarray(T_d)
+    best = np.zeros(R_d.size, dtype=np.int64)
+    for s in range(0, R_d.size, 256):
+        r = R_d[s:s + 256, None]
+        tt = T_d[s:s + 256, None]
+        c = ((pR[None, :] - r) / r) ** 2 + ((pT[None, :] - tt) / np.maximum(tt, 1e-6)) ** 2
+        best[s:s + 256] = np.argmin(c, axis=1)
+    x0 = np.stack([A.ravel()[best], Sg.ravel()[best]], 1) * math.log(10)   # ln mu_a, ln mu_s
+    if polish:
+        def ev(points, ids):
+            P = np.array(points)
+            pr = sur.predict(np.exp(P[:, 0]), np.exp(P[:, 1]), g, n, t)
+            i = np.array(ids)
+            return (((pr["R_d"] - R_d[i]) / R_d[i]) ** 2 +
+                    ((pr["T_d"] - T_d[i]) / np.maximum(T_d[i], 1e-6)) ** 2)
+        res = _praxis.minimize_batch(list(x0), ev, 1e-4, 0.05, max_evals=200)
+        x0 = np.array([r[0] for r in res])
+    mua = np.exp(x0[:, 0])
+    musr = np.exp(x0[:, 1]) * (1 - g)
+    return np.stack([mua, musr], 1)
+
+
+class LogCost(M.BatchCost):
+    """MC cost of points given as (ln mu_a, ln mu_s')."""
+
+    def __call__(self, points, ids):
+        lin = [np.array([math.exp(p[0]), math.exp(p[1])]) for p in points]
+        return super().__call__(lin, ids)
+
+
+def refine_batch(problems, seeds, packets=100_000, step=0.1, tol=2e-3, seed=7,
+                 engine: Optional[Engine] = None, max_evals=400, stream_base=0):
+    """Batched PRAXIS on the MC cost in log coordinates from `seeds`
+    ((mu_a, mu_s') per problem).  Returns (solutions [P, 2], evaluations, stats)."""
+    eng = engine or default_engine()
+    x0 = [np.log(np.asarray(s, dtype=np.float64)) for s in seeds]
+    cost = LogCost(eng, problems, packets, seed=seed,
+                   streams=np.arange(len(problems)) + stream_base)
+    res = _praxis.minimize_batch(x0, cost, tol, step, max_evals=max_evals)
+    sol = np.array([np.exp(r[0]) for r in res])
+    nev = np.array([r[2] for r in res])
+    return sol, nev, {"launches": cost.launches, "packets": cost.packets, "kernel_ms": cost.kernel_ms}
+
+
+def check(problems, sol, packets=1_000_000, seed=9, engine: Optional[Engine] = None, miss_tol=0.01):
+    """Independent MC at each solution; relative misfit of R_d and T_d."""
+    eng = engine or default_engine()
+    cfgs = [p.configs(float(a), float(s))[0][0] for p, (a, s) in zip(problems, sol)]
+    res = eng.simulate_batch(cfgs, packets, seed=seed, streams=np.arange(len(cfgs), dtype=np.uint32) + 90_000_000)
+    rR = np.array([abs(r.mean["R_d"] - p.measured.R_d) / max(p.measured.R_d, 1e-12) for r, p in zip(res, problems)])
+    rT = np.array([abs(r.mean["T_d"] - p.measured.T_d) / max(p.measured.T_d, 1e-12) for r, p in zip(res, problems)])
+    miss = (rR > miss_tol) | (rT > miss_tol)
+    return miss, rR, rT, len(cfgs) * packets
+
+
+def solve_plain(problem_, guess=(0.3, 5.0), packets=1_000_000, step=5.0, tol=0.1, seed=3,
+                engine: Optional[Engine] = None, stream=0, max_evals=2000):
+    """The reference algorithm: PRAXIS in (mu_a, mu_s'), one launch per evaluation."""
+    t0 = time.perf_counter()
+    x, fx, nf, cost = M.solve_sequential(problem_, guess, n_packets=packets, step=step, tol=tol,
+                                         seed=seed, engine=engine, max_evals=max_evals, stream=stream)
+    return x, nf, time.perf_counter() - t0, cost
+
+
+def errors(sol, truth_mua, truth_musr):
+    ea = np.abs(sol[:, 0] - truth_mua) / truth_mua
+    es = np.abs(sol[:, 1] - truth_musr) / truth_musr
+    return ea, es
+
+
+def summary(ea, es):
+    return {"median_mu_a": float(np.median(ea)), "median_mus_reduced": float(np.median(es)),
+            "p90_mu_a": float(np.percentile(ea, 90)), "p90_mus_reduced": float(np.percentile(es, 90)),
+            "frac_both_within_1pct": float(np.mean((ea <= 0.01) & (es <= 0.01))),
+            "frac_both_within_2pct": float(np.mean((ea <= 0.02) & (es <= 0.02)))}
+
+
+def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_packets=1_000_000,
+        plain_count=8, plain_packets=1_000_000, seed=0, engine: Optional[Engine] = None,
+        with_ablation=True, log=print):
+    """Surrogate-seeded batched solves vs plain MC minimization on the same GPU."""
+    eng = engine or default_engine()
+    meas = make_measurements(count, meas_packets, seed=seed, engine=eng)
+    probs = [problem(r, t) for r, t in zip(meas["R_d"], meas["T_d"])]
+    out = {"count": count}
+
+    t0 = time.perf_counter()
+    seeds = surrogate_seeds(sur, meas["R_d"], meas["T_d"])
+    t_seed = time.perf_counter() - t0
+    sol, nev, st = refine_batch(probs, seeds, packets=refine_packets, engine=eng)
+    t_ref = time.perf_counter() - t0 - t_seed
+    miss, rR, rT, chk_packets = check(probs, sol, check_packets, engine=eng)
+    t_chk = time.perf_counter() - t0 - t_seed - t_ref
+    n_miss = int(miss.sum())
+    fb_time = 0.0
+    if n_miss:
+        idx = np.where(miss)[0]
+        tf = time.perf_counter()
+        fb, _, _ = refine_batch([probs[i] for i in idx], [(0.3, 5.0)] * n_miss, packets=refine_packets,
+                                step=1.0, tol=2e-3, engine=eng, stream_base=10_000_000)
+        sol[idx] = fb
+        fb_time = time.perf_counter() - tf
+    total = time.perf_counter() - t0
+    ea, es = errors(sol, meas["mu_a"], meas["mus_reduced"])
+    se0 = errors(seeds, meas["mu_a"], meas["mus_reduced"])
+    out["seeded"] = {"solves_per_s": count / total, "seconds": total, "seed_s": t_seed, "refine_s": t_ref,
+                     "check_s": t_chk, "fallback_s": fb_time, "misses": n_miss,
+                     "mean_evals": float(nev.mean()), "launches": st["launches"],
+                     "refine_packets": st["packets"], "check_packets": chk_packets,
+                     "errors": summary(ea, es), "seed_errors": summary(*se0)}
+    log("seeded: %.1f solves/s, %d misses, %.1f evals/solve" % (count / total, n_miss, nev.mean()))
+
+    if with_ablation:
+        t1 = time.perf_counter()
+        sol2, nev2, st2 = refine_batch(probs, [(0.3, 5.0)] * count, packets=refine_packets, step=1.0,
+                                       engine=eng, stream_base=20_000_000)
+        t2 = time.perf_counter() - t1
+        ea2, es2 = errors(sol2, meas["mu_a"], meas["mus_reduced"])
+        out["batched_unseeded"] = {"solves_per_s": count / t2, "seconds": t2,
+                                   "mean_evals": float(nev2.mean()), "errors": summary(ea2, es2)}
+        log("batched without surrogate: %.1f solves/s, %.1f evals/solve" % (count / t2, nev2.mean()))
+
+    pe, ps, ptimes, pevals = [], [], [], []
+    for k in range(plain_count):
+        x, nf, secs, _ = solve_plain(probs[k], packets=plain_packets, engine=eng, stream=30_000_000 + k)
+        pe.append(x)
+        ptimes.append(secs)
+        pevals.append(nf)
+        log("plain solve %d: %.2f s, %d evals" % (k, secs, nf))
+    pe = np.array(pe)
+    ea3, es3 = errors(pe, meas["mu_a"][:plain_count], meas["mus_reduced"][:plain_count])
+    out["plain"] = {"solves_per_s": plain_count / sum(ptimes), "seconds": sum(ptimes),
+                    "count": plain_count, "mean_evals": float(np.mean(pevals)),
+                    "errors": summary(ea3, es3)}
+    out["speedup_seeded_vs_plain"] = out["seeded"]["solves_per_s"] / out["plain"]["solves_per_s"]
+    return out

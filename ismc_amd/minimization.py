@@ -83,16 +83,21 @@ class Problem:
         return cfgs, src
 
 
+REL_FLOOR = 1e-6
+
+
 def cost_from(values, measured: Measured):
     """sqrt(((R_d - M_R)/M_R)^2 + (R_s - M_S)^2 + ((T_d - M_T)/M_T)^2 + (T_u - M_U)^2)
-    over the measured terms (ISMLMC-minimization.cu:119-129)."""
+    over the measured terms (ISMLMC-minimization.cu:119-129).  The relative
+    terms divide by max(M, 1e-6): a measured 0 (a thick absorbing sample)
+    would divide by zero in the reference."""
     c = 0.0
     if measured.R_d is not None:
-        c += ((values["R_d"] - measured.R_d) / measured.R_d) ** 2
+        c += ((values["R_d"] - measured.R_d) / max(measured.R_d, REL_FLOOR)) ** 2
     if measured.R_s is not None:
         c += (values["R_s"] - measured.R_s) ** 2
     if measured.T_d is not None:
-        c += ((values["T_d"] - measured.T_d) / measured.T_d) ** 2
+        c += ((values["T_d"] - measured.T_d) / max(measured.T_d, REL_FLOOR)) ** 2
     if measured.T_u is not None:
         c += (values["T_u"] - measured.T_u) ** 2
     return math.sqrt(c)

@@ -1,0 +1,69 @@
+"""Generate MC-labelled data, train the surrogate, and run the inverse
+benchmark on one GPU.
+
+    python -m ismc_amd.train_surrogate --samples 262144 --packets 20000 \\
+        --epochs 60 --out weights.npz [--inverse 4096] [--json result.json]
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import inverse, surrogate
+from .engine import Engine
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--samples", type=int, default=1 << 18)
+    ap.add_argument("--packets", type=int, default=20000)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default="")
+    ap.add_argument("--load", default="", help="skip training, load weights")
+    ap.add_argument("--inverse", type=int, default=0, help="measurements for the inverse benchmark")
+    ap.add_argument("--plain", type=int, default=8)
+    ap.add_argument("--refine-packets", type=int, default=100000)
+    ap.add_argument("--json", default="")
+    a = ap.parse_args(argv)
+    log = lambda *x: print(*x, file=sys.stderr, flush=True)
+    eng = Engine(0)
+    result = {}
+    if a.load:
+        sur = surrogate.Surrogate.load(a.load)
+    else:
+        log("generating %d samples x %d packets" % (a.samples, a.packets))
+        inp, tal, secs, packets = surrogate.generate_dataset(
+            a.samples, a.packets, seed=a.seed, engine=eng,
+            progress=lambda d, n, t: log("  labels %d/%d  %.1f s" % (d, n, t)))
+        result["dataset"] = {"samples": a.samples, "packets_per_sample": a.packets,
+                             "seconds": secs, "packets_per_s": packets / secs}
+        log("dataset: %.1f s (%.3g packets/s)" % (secs, packets / secs))
+        t0 = time.perf_counter()
+        sur, info = surrogate.train_surrogate(inp, tal, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                                              seed=a.seed, lr_decay_every=max(1, a.epochs // 3))
+        info["wall_s"] = time.perf_counter() - t0
+        # hold-out error in units of the labels' own MC standard error
+        info["label_se_R_d_median"] = float(np.median(np.sqrt(tal[:, 0] * (1 - tal[:, 0]) / a.packets)))
+        result["training"] = {k: v for k, v in info.items() if k != "losses"}
+        result["training"]["loss_first_last"] = [info["losses"][0], info["losses"][-1]]
+        log("training: %s" % json.dumps(result["training"]))
+        if a.out:
+            sur.train_info = result["training"]
+            sur.save(a.out)
+    if a.inverse:
+        result["inverse"] = inverse.run(sur, count=a.inverse, plain_count=a.plain,
+                                        refine_packets=a.refine_packets, engine=eng, log=log)
+    out = json.dumps(result)
+    print(out, flush=True)
+    if a.json:
+        with open(a.json, "w") as f:
+            f.write(out)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,36 @@
+"""Inverse pipeline host logic (CPU) with an analytic stand-in for the GPU
+transport and for the surrogate."""
+import numpy as np
+
+from ismc_amd import inverse
+from test_minimization_host import AnalyticEngine
+
+
+class AnalyticSurrogate:
+    def predict(self, mua, mus, g, n, t):
+        mua = np.atleast_1d(mua)
+        mus = np.atleast_1d(mus)
+        s = mus * (1 - g)
+        Rd = 0.5 * s / (s + 8 * mua + 5)
+        Td = 0.6 * np.exp(-0.08 * mua - 0.01 * s)
+        return {"R_d": Rd, "T_d": Td, "T_u": 0 * Rd, "R_tot": Rd}
+
+
+def test_seed_refine_check_on_analytic_model():
+    eng = AnalyticEngine()
+    truths = np.array([(0.05, 2.0), (0.5, 20.0), (3.0, 80.0), (0.2, 150.0)])
+    probs = []
+    for a, s in truths:
+        t = eng.tallies(inverse.problem(0.5, 0.5).configs(a, s)[0][0])
+        probs.append(inverse.problem(t["R_d"], t["T_d"]))
+    R = [p.measured.R_d for p in probs]
+    T = [p.measured.T_d for p in probs]
+    seeds = inverse.surrogate_seeds(AnalyticSurrogate(), R, T, grid=64)
+    assert np.allclose(seeds, truths, rtol=1e-3)
+    sol, nev, st = inverse.refine_batch(probs, seeds * 1.05, packets=1000, engine=eng, tol=1e-6)
+    assert np.allclose(sol, truths, rtol=1e-4)
+    miss, rR, rT, _ = inverse.check(probs, sol, 1000, engine=eng)
+    assert not miss.any() and rR.max() < 1e-4
+    ea, es = inverse.errors(sol, truths[:, 0], truths[:, 1])
+    s = inverse.summary(ea, es)
+    assert s["frac_both_within_1pct"] == 1.0
