@@ -259,6 +259,8 @@ struct ismc_mlp {
   float* Tb = nullptr;                  // batch targets
   float* lossp = nullptr;
   int* idx = nullptr;
+  float* stage_x = nullptr;             // [maxB][n_in] upload staging
+  float* stage_y = nullptr;             // [maxB][n_out]
   float* ds_x = nullptr;                // resident dataset
   float* ds_y = nullptr;
   int64_t ds_n = 0;
@@ -304,7 +306,7 @@ static int gemm(ismc_mlp* p, int M, int N, int K, const bf16_t* A, int lda, cons
 
 static void mlp_free(ismc_mlp* p) {
   void* bufs[] = {p->w, p->g, p->m, p->v, p->mask, p->wb, p->X0, p->Y, p->dA, p->dB,
-                  p->slabs, p->Tb, p->lossp, p->idx, p->ds_x, p->ds_y};
+                  p->slabs, p->Tb, p->lossp, p->idx, p->ds_x, p->ds_y, p->stage_x, p->stage_y};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto z : p->Z)
@@ -373,17 +375,11 @@ static int backward(ismc_mlp* p, int B, float* loss_host) {
 }
 
 static int upload_batch(ismc_mlp* p, const float* x, const float* y, int B) {
-  // stage through the dataset buffers: rows 0..B-1 with the identity order
+  // stage through persistent device buffers, rows 0..B-1 in order
   std::vector<int> id(B);
   for (int i = 0; i < B; ++i) id[i] = i;
-  float* tx = nullptr;
-  float* ty = nullptr;
-  int rc;
-  if ((rc = dalloc(p, &tx, (size_t)B * p->n_in))) return rc;
-  if (y && (rc = dalloc(p, &ty, (size_t)B * p->n_out))) {
-    (void)hipFree(tx);
-    return rc;
-  }
+  float* tx = p->stage_x;
+  float* ty = y ? p->stage_y : nullptr;
   hipError_t e = hipMemcpyAsync(tx, x, sizeof(float) * B * p->n_in, hipMemcpyHostToDevice, p->stream);
   if (e == hipSuccess && y)
     e = hipMemcpyAsync(ty, y, sizeof(float) * B * p->n_out, hipMemcpyHostToDevice, p->stream);
@@ -395,8 +391,6 @@ static int upload_batch(ismc_mlp* p, const float* x, const float* y, int B) {
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-  (void)hipFree(tx);
-  if (ty) (void)hipFree(ty);
   if (e != hipSuccess) return mlp_err(p, ISMC_EDEVICE, "batch upload", e);
   return 0;
 }
@@ -462,6 +456,8 @@ int ismc_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t depth, int
   rc = rc ? rc : dalloc(p, &p->Tb, B * n_out);
   rc = rc ? rc : dalloc(p, &p->lossp, (B + 255) / 256);
   rc = rc ? rc : dalloc(p, &p->idx, B);
+  rc = rc ? rc : dalloc(p, &p->stage_x, B * n_in);
+  rc = rc ? rc : dalloc(p, &p->stage_y, B * n_out);
   p->Z.assign(depth, nullptr);
   p->A.assign(depth, nullptr);
   for (int l = 0; l < depth && !rc; ++l) {

@@ -27,7 +27,8 @@ def test_seed_refine_check_on_analytic_model():
     T = [p.measured.T_d for p in probs]
     seeds = inverse.surrogate_seeds(AnalyticSurrogate(), R, T, grid=64)
     assert np.allclose(seeds, truths, rtol=1e-3)
-    sol, nev, st = inverse.refine_batch(probs, seeds * 1.05, packets=1000, engine=eng, tol=1e-6)
+    sol, nev, st = inverse.refine_batch(probs, seeds * 1.05, packets=1000, engine=eng, tol=1e-6,
+                                        step=0.1, max_evals=400)
     assert np.allclose(sol, truths, rtol=1e-4)
     miss, rR, rT, _ = inverse.check(probs, sol, 1000, engine=eng)
     assert not miss.any() and rR.max() < 1e-4
