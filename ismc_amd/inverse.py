@@ -90,12 +90,29 @@ def surrogate_seeds(sur, R_d, T_d, g=G, n=N_INDEX, t=THICK, grid=256, polish=Tru
     return np.stack([mua, musr], 1)
 
 
+# search box in (ln mu_a, ln mu_s'); outside it the cost is a penalty
+LOG_BOX = np.array([[math.log(1e-4), math.log(1e3)], [math.log(1e-3), math.log(1e4)]])
+
+
 class LogCost(M.BatchCost):
-    """MC cost of points given as (ln mu_a, ln mu_s')."""
+    """MC cost of points given as (ln mu_a, ln mu_s'); points outside
+    LOG_BOX get PENALTY plus their distance to it."""
 
     def __call__(self, points, ids):
-        lin = [np.array([math.exp(p[0]), math.exp(p[1])]) for p in points]
-        return super().__call__(lin, ids)
+        out = np.zeros(len(points))
+        inside, ins_ids, slots = [], [], []
+        for k, (p, i) in enumerate(zip(points, ids)):
+            p = np.asarray(p, dtype=np.float64)
+            over = np.maximum(LOG_BOX[:, 0] - p, 0) + np.maximum(p - LOG_BOX[:, 1], 0)
+            if not np.all(np.isfinite(p)) or np.any(over > 0):
+                out[k] = M.PENALTY + float(np.sum(np.nan_to_num(over, nan=1e6)))
+            else:
+                inside.append(np.exp(p))
+                ins_ids.append(i)
+                slots.append(k)
+        if inside:
+            out[slots] = super().__call__(inside, ins_ids)
+        return out
 
 
 def refine_batch(problems, seeds, packets=100_000, step=0.03, tol=3e-3, seed=7,
@@ -119,14 +136,75 @@ def refine_batch(problems, seeds, packets=100_000, step=0.03, tol=3e-3, seed=7,
     return sol, nev, {"launches": cost.launches, "packets": cost.packets, "kernel_ms": cost.kernel_ms}
 
 
-def check(problems, sol, packets=1_000_000, seed=9, engine: Optional[Engine] = None, miss_tol=0.01):
-    """Independent MC at each solution; relative misfit of R_d and T_d."""
+def refine_newton(problems, seeds, sur, iters=4, packets=100_000, seed=7, engine: Optional[Engine] = None,
+                  stream_base=0, g=G, n=N_INDEX, t=THICK, max_step=0.5, log=None):
+    """Defect correction with the surrogate's Jacobian: in y = (ln mu_a, ln mu_s),
+    y <- y - J^-1 r with r = (MC(y) - M) / M (relative residuals of R_d, T_d,
+    every solve's MC in one launch per iteration, common random numbers per
+    solve) and J the surrogate's relative Jacobian by central differences.
+    The steps shrink geometrically when the surrogate's slopes are close to
+    the transport's.  Returns (solutions, evaluations per solve, stats)."""
+    eng = engine or default_engine()
+    P = len(problems)
+    Mv = np.array([[p.measured.R_d, p.measured.T_d] for p in problems])
+    Mv = np.maximum(Mv, 1e-6)
+    y = np.stack([np.log(np.asarray(seeds)[:, 0]), np.log(np.asarray(seeds)[:, 1] / (1 - g))], 1)
+    streams = np.arange(P) + stream_base
+    stats = {"launches": 0, "packets": 0, "kernel_ms": 0.0}
+    h = 1e-3
+    for it in range(iters):
+        mua, mus = np.exp(y[:, 0]), np.exp(y[:, 1])
+        cfgs = [p.configs(float(a), float(s * (1 - g)))[0][0] for p, a, s in zip(problems, mua, mus)]
+        res = eng.simulate_batch(cfgs, packets, seed=seed, streams=streams, use_graph=True)
+        stats["launches"] += 1
+        stats["packets"] += packets * P
+        stats["kernel_ms"] += res[0].kernel_ms
+        F = np.array([[r.mean["R_d"], r.mean["T_d"]] for r in res])
+        r = (F - Mv) / Mv
+        # surrogate Jacobian d(R, T)/dy by central differences, all solves at once
+        pts = np.concatenate([y + [h, 0], y - [h, 0], y + [0, h], y - [0, h]])
+        pr = sur.predict(np.exp(pts[:, 0]), np.exp(pts[:, 1]), g, n, t)
+        V = np.stack([pr["R_d"], pr["T_d"]], 1).reshape(4, P, 2)
+        J = np.stack([(V[0] - V[1]) / (2 * h), (V[2] - V[3]) / (2 * h)], 2) / Mv[:, :, None]
+        det = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
+        ok = np.abs(det) > 1e-12
+        inv = np.zeros_like(J)
+        inv[ok, 0, 0] = J[ok, 1, 1] / det[ok]
+        inv[ok, 1, 1] = J[ok, 0, 0] / det[ok]
+        inv[ok, 0, 1] = -J[ok, 0, 1] / det[ok]
+        inv[ok, 1, 0] = -J[ok, 1, 0] / det[ok]
+        dy = -np.einsum("pij,pj->pi", inv, r)
+        nrm = np.linalg.norm(dy, axis=1)
+        dy *= np.minimum(1.0, max_step / np.maximum(nrm, 1e-300))[:, None]
+        y = np.clip(y + dy, [[-9.2, -6.9]], [[6.9, 9.2]])
+        if log:
+            log("    defect correction %d: median |r| %.2e, median |dy| %.2e" %
+                (it, float(np.median(np.abs(r).max(1))), float(np.median(nrm))))
+    sol = np.stack([np.exp(y[:, 0]), np.exp(y[:, 1]) * (1 - g)], 1)
+    return sol, np.full(P, iters), stats
+
+
+def check(problems, sol, packets=1_000_000, seed=9, engine: Optional[Engine] = None,
+          refine_packets=100_000, z=4.0, floor=0.002):
+    """Independent MC at each solution.  A solution misses when R_d or T_d
+    differs from the measurement by more than z combined MC standard errors
+    (this check's and the refinement's, which scales as sqrt(packets /
+    refine_packets)) plus a relative floor.  Returns (miss, relative
+    misfits R, T, packets)."""
     eng = engine or default_engine()
     cfgs = [p.configs(float(a), float(s))[0][0] for p, (a, s) in zip(problems, sol)]
     res = eng.simulate_batch(cfgs, packets, seed=seed, streams=np.arange(len(cfgs), dtype=np.uint32) + 90_000_000)
-    rR = np.array([abs(r.mean["R_d"] - p.measured.R_d) / max(p.measured.R_d, 1e-12) for r, p in zip(res, problems)])
-    rT = np.array([abs(r.mean["T_d"] - p.measured.T_d) / max(p.measured.T_d, 1e-12) for r, p in zip(res, problems)])
-    miss = (rR > miss_tol) | (rT > miss_tol)
+    k = math.sqrt(1.0 + packets / float(refine_packets))
+    miss = np.zeros(len(cfgs), dtype=bool)
+    rR = np.zeros(len(cfgs))
+    rT = np.zeros(len(cfgs))
+    for j, (r, p) in enumerate(zip(res, problems)):
+        for key, arr in (("R_d", rR), ("T_d", rT)):
+            meas = getattr(p.measured, key)
+            d = abs(r.mean[key] - meas)
+            arr[j] = d / max(meas, 1e-12)
+            if d > z * k * r.se[key] + floor * meas:
+                miss[j] = True
     return miss, rR, rT, len(cfgs) * packets
 
 
@@ -158,66 +236,83 @@ def summary(ea, es):
             "frac_both_within_2pct": float(np.mean((ea <= 0.02) & (es <= 0.02)))}
 
 
-def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_packets=1_000_000,
-        plain_count=8, plain_packets=1_000_000, seed=0, engine: Optional[Engine] = None,
-        with_ablation=True, log=print):
-    """Surrogate-seeded batched solves vs plain MC minimization on the same GPU."""
-    eng = engine or default_engine()
-    meas = make_measurements(count, meas_packets, seed=seed, engine=eng)
-    probs = [problem(r, t) for r, t in zip(meas["R_d"], meas["T_d"])]
-    out = {"count": count}
-
-    log("measurements ready (%d at %d packets)" % (count, meas_packets))
+def _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng, log, stream_base):
     t0 = time.perf_counter()
-    seeds = surrogate_seeds(sur, meas["R_d"], meas["T_d"])
-    t_seed = time.perf_counter() - t0
-    log("seeds ready (%.1f s); refining" % t_seed)
-    sol, nev, st = refine_batch(probs, seeds, packets=refine_packets, engine=eng, log=log)
-    t_ref = time.perf_counter() - t0 - t_seed
-    miss, rR, rT, chk_packets = check(probs, sol, check_packets, engine=eng)
-    t_chk = time.perf_counter() - t0 - t_seed - t_ref
+    if method == "newton":
+        sol, nev, st = refine_newton(probs, seeds, sur, packets=refine_packets, engine=eng,
+                                     stream_base=stream_base, log=log)
+    else:
+        sol, nev, st = refine_batch(probs, seeds, packets=refine_packets, engine=eng, log=log,
+                                    stream_base=stream_base)
+    t_ref = time.perf_counter() - t0
+    miss, rR, rT, chk_packets = check(probs, sol, check_packets, engine=eng, refine_packets=refine_packets)
+    t_chk = time.perf_counter() - t0 - t_ref
     n_miss = int(miss.sum())
     fb_time = 0.0
     if n_miss:
         idx = np.where(miss)[0]
+        log("  fallback: plain batched MC minimization for %d misses" % n_miss)
         tf = time.perf_counter()
-        log("fallback for %d misses" % n_miss)
         fb, _, _ = refine_batch([probs[i] for i in idx], [GENERIC_GUESS] * n_miss, packets=refine_packets,
-                                step=1.0, tol=3e-3, max_evals=400, engine=eng, stream_base=10_000_000, log=log)
+                                step=1.0, tol=3e-3, max_evals=400, engine=eng,
+                                stream_base=stream_base + 10_000_000, log=log)
         sol[idx] = fb
         fb_time = time.perf_counter() - tf
-    total = time.perf_counter() - t0
-    ea, es = errors(sol, meas["mu_a"], meas["mus_reduced"])
-    se0 = errors(seeds, meas["mu_a"], meas["mus_reduced"])
-    out["seeded"] = {"solves_per_s": count / total, "seconds": total, "seed_s": t_seed, "refine_s": t_ref,
-                     "check_s": t_chk, "fallback_s": fb_time, "misses": n_miss,
-                     "mean_evals": float(nev.mean()), "launches": st["launches"],
-                     "refine_packets": st["packets"], "check_packets": chk_packets,
-                     "errors": summary(ea, es), "seed_errors": summary(*se0)}
-    log("seeded: %.1f solves/s, %d misses, %.1f evals/solve" % (count / total, n_miss, nev.mean()))
+    return sol, {"refine_s": t_ref, "check_s": t_chk, "fallback_s": fb_time, "misses": n_miss,
+                 "mean_evals": float(np.mean(nev)), "launches": st["launches"],
+                 "refine_packets": st["packets"], "check_packets": chk_packets,
+                 "misfit_R_median": float(np.median(rR)), "misfit_T_median": float(np.median(rT))}
 
+
+def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_packets=1_000_000,
+        plain_count=8, plain_packets=1_000_000, seed=0, engine: Optional[Engine] = None,
+        methods=("newton", "praxis"), with_ablation=True, log=print):
+    """Surrogate-seeded batched solves vs plain MC minimization on the same
+    GPU, over `count` synthetic measurements (1 % noise on R_d and T_d)."""
+    eng = engine or default_engine()
+    meas = make_measurements(count, meas_packets, seed=seed, engine=eng)
+    probs = [problem(r, t) for r, t in zip(meas["R_d"], meas["T_d"])]
+    out = {"count": count, "meas_packets": meas_packets, "refine_packets": refine_packets,
+           "check_packets": check_packets}
+    log("measurements ready (%d at %d packets)" % (count, meas_packets))
+    t0 = time.perf_counter()
+    seeds = surrogate_seeds(sur, meas["R_d"], meas["T_d"])
+    t_seed = time.perf_counter() - t0
+    out["seed_s"] = t_seed
+    out["seed_errors"] = summary(*errors(seeds, meas["mu_a"], meas["mus_reduced"]))
+    log("seeds ready (%.1f s): %s" % (t_seed, out["seed_errors"]))
+    for k, method in enumerate(methods):
+        log("seeded refinement: %s" % method)
+        sol, info = _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng, log,
+                                  stream_base=k * 40_000_000)
+        total = t_seed + info["refine_s"] + info["check_s"] + info["fallback_s"]
+        info["seconds"] = total
+        info["solves_per_s"] = count / total
+        info["errors"] = summary(*errors(sol, meas["mu_a"], meas["mus_reduced"]))
+        out["seeded_" + method] = info
+        log("  %s: %.1f solves/s, %d misses, errors %s" % (method, count / total, info["misses"], info["errors"]))
     if with_ablation:
+        log("ablation: batched PRAXIS from the generic guess (no surrogate)")
         t1 = time.perf_counter()
-        log("ablation: batched refinement from the generic guess")
         sol2, nev2, st2 = refine_batch(probs, [GENERIC_GUESS] * count, packets=refine_packets, step=1.0,
-                                       max_evals=400, engine=eng, stream_base=20_000_000, log=log)
+                                       max_evals=400, engine=eng, stream_base=100_000_000, log=log)
         t2 = time.perf_counter() - t1
-        ea2, es2 = errors(sol2, meas["mu_a"], meas["mus_reduced"])
         out["batched_unseeded"] = {"solves_per_s": count / t2, "seconds": t2,
-                                   "mean_evals": float(nev2.mean()), "errors": summary(ea2, es2)}
-        log("batched without surrogate: %.1f solves/s, %.1f evals/solve" % (count / t2, nev2.mean()))
-
-    pe, ps, ptimes, pevals = [], [], [], []
+                                   "mean_evals": float(nev2.mean()),
+                                   "errors": summary(*errors(sol2, meas["mu_a"], meas["mus_reduced"]))}
+        log("  unseeded: %.1f solves/s, %.1f evals/solve" % (count / t2, nev2.mean()))
+    pe, ptimes, pevals = [], [], []
     for k in range(plain_count):
-        x, nf, secs, _ = solve_plain(probs[k], packets=plain_packets, engine=eng, stream=30_000_000 + k)
+        x, nf, secs, _ = solve_plain(probs[k], packets=plain_packets, engine=eng, stream=120_000_000 + k)
         pe.append(x)
         ptimes.append(secs)
         pevals.append(nf)
-        log("plain solve %d: %.2f s, %d evals" % (k, secs, nf))
+        log("  plain solve %d: %.2f s, %d evaluations" % (k, secs, nf))
     pe = np.array(pe)
-    ea3, es3 = errors(pe, meas["mu_a"][:plain_count], meas["mus_reduced"][:plain_count])
     out["plain"] = {"solves_per_s": plain_count / sum(ptimes), "seconds": sum(ptimes),
                     "count": plain_count, "mean_evals": float(np.mean(pevals)),
-                    "errors": summary(ea3, es3)}
-    out["speedup_seeded_vs_plain"] = out["seeded"]["solves_per_s"] / out["plain"]["solves_per_s"]
+                    "packets_per_eval": plain_packets,
+                    "errors": summary(*errors(pe, meas["mu_a"][:plain_count], meas["mus_reduced"][:plain_count]))}
+    for method in methods:
+        out["speedup_%s_vs_plain" % method] = out["seeded_" + method]["solves_per_s"] / out["plain"]["solves_per_s"]
     return out

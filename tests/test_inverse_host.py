@@ -35,3 +35,16 @@ def test_seed_refine_check_on_analytic_model():
     ea, es = inverse.errors(sol, truths[:, 0], truths[:, 1])
     s = inverse.summary(ea, es)
     assert s["frac_both_within_1pct"] == 1.0
+
+
+def test_defect_correction_converges_with_an_exact_model():
+    eng = AnalyticEngine()
+    truths = np.array([(0.05, 2.0), (0.5, 20.0), (3.0, 80.0)])
+    probs = []
+    for a, s in truths:
+        t = eng.tallies(inverse.problem(0.5, 0.5).configs(a, s)[0][0])
+        probs.append(inverse.problem(t["R_d"], t["T_d"]))
+    sol, nev, st = inverse.refine_newton(probs, truths * [1.2, 0.85], AnalyticSurrogate(), iters=6,
+                                         packets=1000, engine=eng)
+    assert np.allclose(sol, truths, rtol=1e-5)
+    assert st["launches"] == 6 and eng.calls == [3] * 6
