@@ -334,11 +334,11 @@ static int forward(ismc_mlp* p, int B) {
 }
 
 // loss + backward on the current batch (targets in Tb); gradients into g
-static int backward(ismc_mlp* p, int B, float* loss_host) {
+static int backward(ismc_mlp* p, int B, float* loss_host, float* loss_dev = nullptr) {
   int rc;
   const MlpLayer& Lo = p->L[p->depth];
   const int nb = (B + 255) / 256;
-  mse_grad<<<nb, 256, 0, p->stream>>>(p->Y, Lo.N, p->Tb, p->n_out, B, p->dA, p->lossp);
+  mse_grad<<<nb, 256, 0, p->stream>>>(p->Y, Lo.N, p->Tb, p->n_out, B, p->dA, loss_dev ? loss_dev : p->lossp);
   MCHECK(p, hipGetLastError());
   bf16_t* dcur = p->dA;
   bf16_t* dnext = p->dB;
@@ -613,7 +613,9 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
   if ((int64_t)steps_per_epoch * B > p->ds_n) return mlp_err(p, ISMC_EINVAL, "epoch larger than the dataset");
   MCHECK(p, hipSetDevice(p->device));
   const int nb = (B + 255) / 256;
-  std::vector<float> lp(nb);
+  std::vector<float> lp((size_t)nb * steps_per_epoch);
+  float* lossd = nullptr;
+  if ((rc = dalloc(p, &lossd, (size_t)nb * steps_per_epoch))) return rc;
   float ms_total = 0.f;
   for (int e = 0; e < epochs; ++e) {
     MCHECK(p, hipMemcpyAsync(p->idx, perm + (size_t)e * steps_per_epoch * B,
@@ -624,8 +626,10 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
       gather_batch<<<nb, 256, 0, p->stream>>>(p->ds_x, p->n_in, p->ds_y, p->n_out, p->idx + (size_t)s * B, B,
                                               p->in_pad, p->X0, p->Tb);
       MCHECK(p, hipGetLastError());
-      if ((rc = forward(p, B))) return rc;
-      if ((rc = backward(p, B, nullptr))) return rc;
+      if ((rc = forward(p, B)) || (rc = backward(p, B, nullptr, lossd + (size_t)s * nb))) {
+        (void)hipFree(lossd);
+        return rc;
+      }
       p->steps++;
       const float bc1 = 1.f - powf(b1, (float)p->steps), bc2 = 1.f - powf(b2, (float)p->steps);
       const float lr_t = lr * sqrtf(bc2) / bc1;
@@ -639,19 +643,17 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
             (size_t)Ly.N, lr_t, b1, b2, eps);
       }
       MCHECK(p, hipGetLastError());
-      MCHECK(p, hipMemcpyAsync(lp.data(), p->lossp, sizeof(float) * nb, hipMemcpyDeviceToHost, p->stream));
-      MCHECK(p, hipStreamSynchronize(p->stream));
-      double s2 = 0;
-      for (float x : lp) s2 += x;
-      esum += s2 / ((double)B * p->n_out);
     }
     MCHECK(p, hipEventRecord(p->e1, p->stream));
+    MCHECK(p, hipMemcpyAsync(lp.data(), lossd, sizeof(float) * lp.size(), hipMemcpyDeviceToHost, p->stream));
     MCHECK(p, hipStreamSynchronize(p->stream));
+    for (float x : lp) esum += x / ((double)B * p->n_out);
     float ms = 0.f;
     MCHECK(p, hipEventElapsedTime(&ms, p->e0, p->e1));
     ms_total += ms;
     if (epoch_loss) epoch_loss[e] = (float)(esum / steps_per_epoch);
   }
+  (void)hipFree(lossd);
   if (total_ms) *total_ms = ms_total;
   return 0;
 }

@@ -25,7 +25,7 @@ def test_seed_refine_check_on_analytic_model():
         probs.append(inverse.problem(t["R_d"], t["T_d"]))
     R = [p.measured.R_d for p in probs]
     T = [p.measured.T_d for p in probs]
-    seeds = inverse.surrogate_seeds(AnalyticSurrogate(), R, T, grid=64)
+    seeds = inverse.surrogate_seeds(AnalyticSurrogate(), R, T, grid=64, polish=True)
     assert np.allclose(seeds, truths, rtol=1e-3)
     sol, nev, st = inverse.refine_batch(probs, seeds * 1.05, packets=1000, engine=eng, tol=1e-6,
                                         step=0.1, max_evals=400)
