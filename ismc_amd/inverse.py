@@ -136,25 +136,33 @@ def refine_batch(problems, seeds, packets=100_000, step=0.03, tol=3e-3, seed=7,
     return sol, nev, {"launches": cost.launches, "packets": cost.packets, "kernel_ms": cost.kernel_ms}
 
 
-def refine_newton(problems, seeds, sur, iters=4, packets=100_000, seed=7, engine: Optional[Engine] = None,
-                  stream_base=0, g=G, n=N_INDEX, t=THICK, max_step=0.5, log=None):
-    """Defect correction with the surrogate's Jacobian: in y = (ln mu_a, ln mu_s),
-    y <- y - J^-1 r with r = (MC(y) - M) / M (relative residuals of R_d, T_d,
-    every solve's MC in one launch per iteration, common random numbers per
-    solve) and J the surrogate's relative Jacobian by central differences.
-    The steps shrink geometrically when the surrogate's slopes are close to
-    the transport's.  Returns (solutions, evaluations per solve, stats)."""
+def refine_newton(problems, seeds, sur, iters=6, packets=100_000, seed=7, engine: Optional[Engine] = None,
+                  stream_base=0, g=G, n=N_INDEX, t=THICK, radius=0.5, s_cut=0.05, log=None):
+    """Defect correction with the surrogate's Jacobian, in y = (ln mu_a, ln mu_s):
+    r = (MC(y) - M) / M are the relative residuals of R_d, T_d (every solve's
+    MC in one launch per iteration, common random numbers per solve) and J
+    the surrogate's relative Jacobian by central differences.  The step is
+    the Gauss-Newton step -J^+ r with singular values of J below `s_cut`
+    dropped (a direction the network cannot resolve, e.g. mu_a when
+    absorption hardly changes R_d and T_d, is left alone), limited to a
+    trust radius per solve.  A trial whose residual norm is not smaller is
+    rejected and the radius shrinks fourfold; an accepted one doubles it.
+    Returns (best solutions, evaluations per solve, stats)."""
     eng = engine or default_engine()
     P = len(problems)
     Mv = np.array([[p.measured.R_d, p.measured.T_d] for p in problems])
     Mv = np.maximum(Mv, 1e-6)
-    y = np.stack([np.log(np.asarray(seeds)[:, 0]), np.log(np.asarray(seeds)[:, 1] / (1 - g))], 1)
+    y_acc = np.stack([np.log(np.asarray(seeds)[:, 0]), np.log(np.asarray(seeds)[:, 1] / (1 - g))], 1)
+    y_try = y_acc.copy()
+    r_acc = np.full((P, 2), np.inf)
+    rad = np.full(P, float(radius))
     streams = np.arange(P) + stream_base
-    stats = {"launches": 0, "packets": 0, "kernel_ms": 0.0}
+    stats = {"launches": 0, "packets": 0, "kernel_ms": 0.0, "rejected": 0}
     # difference step well above the BF16 network's output resolution (~4e-3)
     h = 0.05
+    lo, hi = np.array([[-9.2, -6.9]]), np.array([[6.9, 9.2]])
     for it in range(iters):
-        mua, mus = np.exp(y[:, 0]), np.exp(y[:, 1])
+        mua, mus = np.exp(y_try[:, 0]), np.exp(y_try[:, 1])
         cfgs = [p.configs(float(a), float(s * (1 - g)))[0][0] for p, a, s in zip(problems, mua, mus)]
         res = eng.simulate_batch(cfgs, packets, seed=seed, streams=streams, use_graph=True)
         stats["launches"] += 1
@@ -162,26 +170,29 @@ def refine_newton(problems, seeds, sur, iters=4, packets=100_000, seed=7, engine
         stats["kernel_ms"] += res[0].kernel_ms
         F = np.array([[r.mean["R_d"], r.mean["T_d"]] for r in res])
         r = (F - Mv) / Mv
-        # surrogate Jacobian d(R, T)/dy by central differences, all solves at once
-        pts = np.concatenate([y + [h, 0], y - [h, 0], y + [0, h], y - [0, h]])
+        better = np.linalg.norm(r, axis=1) < np.linalg.norm(r_acc, axis=1)
+        if it > 0:
+            stats["rejected"] += int((~better).sum())
+        y_acc[better] = y_try[better]
+        r_acc[better] = r[better]
+        rad = np.where(better, np.minimum(rad * 2.0, 1.0), rad * 0.25)
+        if it == iters - 1:
+            break
+        # surrogate Jacobian d(R, T)/dy at the accepted points, all solves at once
+        pts = np.concatenate([y_acc + [h, 0], y_acc - [h, 0], y_acc + [0, h], y_acc - [0, h]])
         pr = sur.predict(np.exp(pts[:, 0]), np.exp(pts[:, 1]), g, n, t)
         V = np.stack([pr["R_d"], pr["T_d"]], 1).reshape(4, P, 2)
         J = np.stack([(V[0] - V[1]) / (2 * h), (V[2] - V[3]) / (2 * h)], 2) / Mv[:, :, None]
-        det = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
-        ok = np.abs(det) > 1e-12
-        inv = np.zeros_like(J)
-        inv[ok, 0, 0] = J[ok, 1, 1] / det[ok]
-        inv[ok, 1, 1] = J[ok, 0, 0] / det[ok]
-        inv[ok, 0, 1] = -J[ok, 0, 1] / det[ok]
-        inv[ok, 1, 0] = -J[ok, 1, 0] / det[ok]
-        dy = -np.einsum("pij,pj->pi", inv, r)
+        U, S, Wt = np.linalg.svd(J)
+        Sinv = np.where(S > s_cut, 1.0 / np.maximum(S, 1e-300), 0.0)
+        dy = -np.einsum("pji,pj,pjk,pk->pi", Wt, Sinv, U, r_acc)
         nrm = np.linalg.norm(dy, axis=1)
-        dy *= np.minimum(1.0, max_step / np.maximum(nrm, 1e-300))[:, None]
-        y = np.clip(y + dy, [[-9.2, -6.9]], [[6.9, 9.2]])
+        dy *= np.minimum(1.0, rad / np.maximum(nrm, 1e-300))[:, None]
+        y_try = np.clip(y_acc + dy, lo, hi)
         if log:
-            log("    defect correction %d: median |r| %.2e, median |dy| %.2e" %
-                (it, float(np.median(np.abs(r).max(1))), float(np.median(nrm))))
-    sol = np.stack([np.exp(y[:, 0]), np.exp(y[:, 1]) * (1 - g)], 1)
+            log("    defect correction %d: median |r| %.2e, median |dy| %.2e, %d rejected" %
+                (it, float(np.median(np.abs(r_acc).max(1))), float(np.median(nrm)), int((~better).sum())))
+    sol = np.stack([np.exp(y_acc[:, 0]), np.exp(y_acc[:, 1]) * (1 - g)], 1)
     return sol, np.full(P, iters), stats
 
 
@@ -251,11 +262,14 @@ def _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng,
     n_miss = int(miss.sum())
     fb_time = 0.0
     if n_miss:
+        # MC minimization from the refined point: the misses are mostly
+        # measurements whose best fit is poorly conditioned (or not exact
+        # within the noise), not wrong basins
         idx = np.where(miss)[0]
-        log("  fallback: plain batched MC minimization for %d misses" % n_miss)
+        log("  fallback: batched MC minimization from the refined point for %d misses" % n_miss)
         tf = time.perf_counter()
-        fb, _, _ = refine_batch([probs[i] for i in idx], [GENERIC_GUESS] * n_miss, packets=refine_packets,
-                                step=1.0, tol=3e-3, max_evals=400, engine=eng,
+        fb, _, _ = refine_batch([probs[i] for i in idx], sol[idx], packets=refine_packets,
+                                step=0.3, tol=3e-3, max_evals=150, engine=eng,
                                 stream_base=stream_base + 10_000_000, log=log)
         sol[idx] = fb
         fb_time = time.perf_counter() - tf

@@ -29,6 +29,8 @@ def main(argv=None):
     ap.add_argument("--plain", type=int, default=8)
     ap.add_argument("--refine-packets", type=int, default=100000)
     ap.add_argument("--json", default="")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the unseeded batched solves")
+    ap.add_argument("--methods", default="newton,praxis")
     a = ap.parse_args(argv)
     log = lambda *x: print(*x, file=sys.stderr, flush=True)
     eng = Engine(0)
@@ -57,7 +59,9 @@ def main(argv=None):
             sur.save(a.out)
     if a.inverse:
         result["inverse"] = inverse.run(sur, count=a.inverse, plain_count=a.plain,
-                                        refine_packets=a.refine_packets, engine=eng, log=log)
+                                        refine_packets=a.refine_packets, engine=eng, log=log,
+                                        methods=tuple(a.methods.split(",")),
+                                        with_ablation=not a.no_ablation)
     out = json.dumps(result)
     print(out, flush=True)
     if a.json:

@@ -44,7 +44,41 @@ def test_defect_correction_converges_with_an_exact_model():
     for a, s in truths:
         t = eng.tallies(inverse.problem(0.5, 0.5).configs(a, s)[0][0])
         probs.append(inverse.problem(t["R_d"], t["T_d"]))
-    sol, nev, st = inverse.refine_newton(probs, truths * [1.2, 0.85], AnalyticSurrogate(), iters=6,
-                                         packets=1000, engine=eng)
+    sol, nev, st = inverse.refine_newton(probs, truths * [1.2, 0.85], AnalyticSurrogate(), iters=8,
+                                         packets=1000, engine=eng, s_cut=1e-6)
     assert np.allclose(sol, truths, rtol=1e-5)
-    assert st["launches"] == 6 and eng.calls == [3] * 6
+    assert st["launches"] == 8 and eng.calls == [3] * 8
+
+
+def test_defect_correction_leaves_unresolved_directions_alone():
+    # at mu_a = 0.05 the model's slope in ln mu_a is ~0.04 (below the cut):
+    # only the mu_s' direction is corrected and the residual still falls
+    eng = AnalyticEngine()
+    truth = np.array([(0.05, 2.0)])
+    t = eng.tallies(inverse.problem(0.5, 0.5).configs(0.05, 2.0)[0][0])
+    probs = [inverse.problem(t["R_d"], t["T_d"])]
+    start = truth * [1.5, 0.8]
+    sol, _, _ = inverse.refine_newton(probs, start, AnalyticSurrogate(), iters=8, packets=1000, engine=eng)
+    def resid(x):
+        v = eng.tallies(probs[0].configs(*x)[0][0])
+        return np.hypot(v["R_d"] / t["R_d"] - 1, v["T_d"] / t["T_d"] - 1)
+    assert resid(sol[0]) < 0.2 * resid(start[0])
+    assert abs(sol[0, 1] / 2.0 - 1) < abs(start[0, 1] / 2.0 - 1)
+
+
+class WrongSlopeSurrogate(AnalyticSurrogate):
+    def predict(self, mua, mus, g, n, t):
+        p = super().predict(mua, mus, g, n, t)
+        # slopes of the wrong sign: every Gauss-Newton step goes uphill
+        return {k: 2 * np.median(v) - v for k, v in p.items()}
+
+
+def test_defect_correction_rejects_uphill_steps():
+    eng = AnalyticEngine()
+    t = eng.tallies(inverse.problem(0.5, 0.5).configs(0.5, 20.0)[0][0])
+    probs = [inverse.problem(t["R_d"], t["T_d"])]
+    start = np.array([(0.6, 17.0)])
+    sol, _, st = inverse.refine_newton(probs, start, WrongSlopeSurrogate(), iters=5, packets=1000,
+                                       engine=eng, s_cut=1e-6)
+    assert np.allclose(sol, start)        # the best point evaluated is the start
+    assert st["rejected"] == 4
