@@ -58,6 +58,45 @@ def cpu_baseline(seconds):
                          "sample": "%d packets, C oracle FP64, OpenMP" % m}}
 
 
+def lookup_sweep(eng):
+    """64 x 64 single-layer lookup grid (mu_a 0.01..10 /cm, mu_s 1..1000 /cm,
+    g 0.8, n 1.40, t 0.1 cm, dual spheres), adaptive packet counts: a 1e4
+    pilot, then N_ij = clamp(Var/(R 0.005)^2, 1e4, 1e6) in one launch.  The
+    first sweep tunes the launch shape; the second is timed."""
+    from ismc_amd import lookup
+    from ismc_amd.workloads import lookup_grid
+    _, _, cfgs = lookup_grid(64, 64)
+    t0 = time.perf_counter()
+    lookup.adaptive_sweep(cfgs, engine=eng, autotune=True)
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res, info = lookup.adaptive_sweep(cfgs, engine=eng, autotune=True, seed=4321)
+    wall = time.perf_counter() - t0
+    kern = (info["pilot_kernel_ms"] + info["main_kernel_ms"]) / 1e3
+    rse = [r["rse_R"] for r in res]
+    return {"points": len(cfgs), "total_packets": info["total_packets"], "wall_s": wall,
+            "kernel_s": kern, "packets_per_s": info["total_packets"] / wall,
+            "kernel_packets_per_s": info["total_packets"] / kern,
+            "max_rse_R": max(rse), "first_sweep_with_tuning_s": t_tune,
+            "tuned": {str(k): v for k, v in eng.tuner.cache.items()}}
+
+
+def inverse_bench(eng, count):
+    """Surrogate-seeded batched inverse solves vs the plain solver (sequential
+    PRAXIS, one 1e6-packet launch per cost evaluation) on this GPU."""
+    from ismc_amd import inverse
+    from ismc_amd.surrogate import Surrogate, packaged_weights
+    sur = Surrogate.load(packaged_weights())
+    r = inverse.run(sur, count=count, plain_count=8, engine=eng, methods=("newton",), with_ablation=False,
+                    log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    n, p = r["seeded_newton"], r["plain"]
+    return {"measurements": count, "noise": "1% on R_d, T_d", "solves_per_s": n["solves_per_s"],
+            "plain_solves_per_s": p["solves_per_s"], "speedup_vs_plain": r["speedup_newton_vs_plain"],
+            "fits_as_well_as_truth": n["fits_as_well_as_truth"],
+            "plain_fits_as_well_as_truth": p["fits_as_well_as_truth"],
+            "errors": n["errors"], "plain_errors": p["errors"], "misses": n["misses"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,12 +106,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--inverse", type=int, default=0,
+                    help="also run this many surrogate-seeded inverse solves against the plain solver")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run a plain HBM copy of this many bytes (byte-counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
-              "(packets shard by packet id across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(packets shard by packet id across GPUs, with one sum of the tallies at the end)", file=sys.stderr)
         sys.exit(2)
 
     from ismc_amd import Engine
@@ -141,6 +182,10 @@ def main():
         n64 = 1 << 26
         r64 = eng.simulate(cfg, n64, seed=78, fp64=True)
         out["fp64_kernel_packets_per_s"] = n64 / (r64.kernel_ms / 1e3)
+    if not a.no_extra:
+        out["lookup_sweep"] = lookup_sweep(eng)
+    if a.inverse:
+        out["inverse"] = inverse_bench(eng, a.inverse)
     if a.copy_probe:
         ms = eng.copy_probe(a.copy_probe)
         out["copy_probe"] = {"bytes": a.copy_probe, "kernel_ms": ms,

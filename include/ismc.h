@@ -88,12 +88,14 @@ typedef struct {
     uint64_t n_aborted;   /* packets stopped by the per-packet event cap */
 } ismc_tally;
 
-/* Launch shape; zero fields mean "use the context's autotuned default". */
+/* Launch shape; zero fields mean "use the context's default".  Tallies do
+   not depend on any of these fields. */
 typedef struct {
     int32_t blocks_per_cu;
     int32_t chunk_packets;   /* packets per work-queue chunk */
     int32_t use_graph;       /* replay through a captured hipGraph when shapes repeat */
-    int32_t _pad;
+    int32_t idle_lanes;      /* lanes of a wave waiting for an entrance, sphere decision
+                                or new packet before the wave serves them (1..64) */
 } ismc_tune;
 
 typedef struct {

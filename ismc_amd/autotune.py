@@ -1,0 +1,107 @@
+"""Launch-shape autotuner for batched transport.
+
+A batched launch is one persistent grid that pulls (configuration, packet
+range) chunks off a work queue.  Three knobs shape it: packets per queue
+chunk (small chunks balance uneven packet counts and short-lived packets;
+large chunks cut queue traffic and per-chunk tally flushes), idle lanes
+(how many lanes of a wave wait for an entrance, sphere decision or new
+packet before the wave serves them: few means the rare code runs nearly
+every step, many means lanes sit idle longer), and workgroups per CU
+(0 = as many as the occupancy calculator allows).
+
+The tallies do not depend on any knob: a packet's random numbers are a
+function of (seed, stream, packet id) only, and the tallies are exact
+fixed-point sums, so every launch shape returns bit-identical results.  The
+tuner therefore only measures: it runs each candidate shape on a bounded
+sample of the workload (the same configurations, packet counts scaled down
+in proportion to a packet budget), one knob at a time (chunk size, then
+idle lanes at the best chunk size, then workgroups per CU), keeps the
+shape with the highest packets per kernel-second, and caches the choice
+per workload signature.
+"""
+import math
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+
+CHUNKS = (512, 1024, 2048, 4096, 8192, 16384, 65536)
+IDLE = (4, 8, 16, 32)
+BLOCKS = (0, 2, 4)
+
+
+def signature(cfgs, n_packets, fp64=False) -> Tuple:
+    """Workload class a tuned shape is reused for: log2 buckets of the
+    point count and of the mean packets per point, the deepest stack, and
+    whether any layer is glass (glass changes the event mix) or FP64."""
+    npk = np.broadcast_to(np.asarray(n_packets, dtype=np.float64), (len(cfgs),))
+    mean = float(npk.mean()) if len(cfgs) else 0.0
+    layers = max((len(c.layers) for c in cfgs), default=0)
+    glass = any(L.mu_s == 0.0 for c in cfgs for L in c.layers)
+    return (int(math.log2(max(len(cfgs), 1))), int(math.log2(max(mean, 1.0))), layers, glass, bool(fp64))
+
+
+def sample_counts(n_packets, npts, budget) -> np.ndarray:
+    """Per-point packet counts of the timing sample: the workload's own
+    counts when they total at most `budget`, else scaled down in proportion
+    (each point keeps at least one packet)."""
+    npk = np.broadcast_to(np.asarray(n_packets, dtype=np.float64), (npts,)).astype(np.float64)
+    total = float(npk.sum())
+    if total <= budget:
+        return npk.astype(np.uint64)
+    return np.maximum(1, np.floor(npk * (budget / total))).astype(np.uint64)
+
+
+class AutoTuner:
+    """Chooses (chunk_packets, idle_lanes, blocks_per_cu) for an Engine's batched launches."""
+
+    def __init__(self, engine, budget_packets: int = 1 << 26, chunks: Sequence[int] = CHUNKS,
+                 blocks: Sequence[int] = BLOCKS, idle: Sequence[int] = IDLE, repeats: int = 2):
+        self.engine = engine
+        self.budget = int(budget_packets)
+        self.chunks = tuple(int(c) for c in chunks)
+        self.blocks = tuple(int(b) for b in blocks)
+        self.idle = tuple(int(i) for i in idle)
+        self.repeats = int(repeats)
+        self.cache: Dict[Tuple, Dict] = {}
+        self.tables: Dict[Tuple, list] = {}
+
+    def params(self, cfgs, n_packets, fp64=False, seed=1234) -> Dict[str, int]:
+        """Keyword arguments for Engine.simulate_batch (tuning on first use)."""
+        key = signature(cfgs, n_packets, fp64)
+        if key not in self.cache:
+            self.cache[key] = self.tune(cfgs, n_packets, fp64=fp64, seed=seed)
+        return dict(self.cache[key])
+
+    def tune(self, cfgs, n_packets, fp64=False, seed=1234) -> Dict[str, int]:
+        npts = len(cfgs)
+        counts = sample_counts(n_packets, npts, self.budget)
+        total = float(counts.sum())
+        streams = np.arange(npts, dtype=np.uint32)
+        max_chunk = int(counts.max()) if npts else 1
+        chunks = [c for c in self.chunks if c <= max(max_chunk, self.chunks[0])]
+        table = []
+        # first launch of a shape pays module load and buffer growth: discard one
+        self.engine.simulate_batch(cfgs, counts, seed=seed, streams=streams, fp64=fp64)
+
+        def timed(shape):
+            for t in table:
+                if all(t[k] == v for k, v in shape.items()):
+                    return t
+            ms = min(self.engine.simulate_batch(cfgs, counts, seed=seed, streams=streams, fp64=fp64,
+                                                **shape)[0].kernel_ms for _ in range(self.repeats))
+            t = dict(shape, kernel_ms=ms, packets_per_s=total / (ms * 1e-3) if ms > 0 else float("inf"))
+            table.append(t)
+            return t
+
+        # start from the library's defaults (occupancy-limited grid, 8 idle lanes)
+        best = {"blocks_per_cu": 0, "chunk_packets": chunks[0],
+                "idle_lanes": 8 if 8 in self.idle else self.idle[0]}
+        for knob, values in (("chunk_packets", chunks), ("idle_lanes", self.idle), ("blocks_per_cu", self.blocks)):
+            rows = [timed(dict(best, **{knob: v})) for v in values]
+            best[knob] = max(rows, key=lambda t: t["packets_per_s"])[knob]
+        self.tables[signature(cfgs, n_packets, fp64)] = table
+        return best
+
+    def report(self, cfgs, n_packets, fp64=False) -> Optional[list]:
+        """Timing table of the signature's tuning run (None if not tuned)."""
+        return self.tables.get(signature(cfgs, n_packets, fp64))

@@ -9,12 +9,17 @@
 // Semantics follow CUDA/CUDAISMLMC.h:99-654 and ISMLMC.py:101-347 +
 // MLMC.py:76-202 of the reference; the structure is our own:
 //
-//  * Event machine.  Each loop iteration advances every lane by exactly
-//    one event of its packet: an entrance (Fresnel at the outer face), a
-//    flight that ends in an interaction (hop, drop, roulette, spin), or a
-//    flight that ends on an interface (Fresnel reflect / refract / exit).
-//    Lanes whose packet ends pick up a new packet at the top of the next
-//    iteration, so a wave never waits for its longest history.
+//  * Event machine.  Each loop iteration advances every lane in flight by
+//    one event of its packet: a flight that ends in an interaction (hop,
+//    drop, roulette, spin) or on an interface (Fresnel reflect / refract /
+//    exit).  The rare steps (an entrance through the outer face, a sphere
+//    decision, the end-of-packet tally, taking a new packet) wait until
+//    enough lanes of the wave need one (the idle-lane threshold) or no
+//    lane is in flight, and then run together in one "rare pass".  With
+//    64 lanes nearly every iteration has some lane at a rare step, so
+//    running them at once instead of on every iteration removes most of
+//    the divergent work.  A wave never waits for its longest history:
+//    finished lanes take new packets at the next rare pass.
 //  * Only z and mu_z are carried.  The slab is laterally infinite and
 //    every tally, boundary distance, Fresnel test and port test depends on
 //    z and mu_z alone; after a spin the new mu_z depends only on (mu_z,
@@ -75,7 +80,7 @@ enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2 };
 template <typename R> struct GlassGuard;
 template <> struct GlassGuard<float> { static constexpr float value = 9.5367431640625e-07f; };  // 2^-20
 template <> struct GlassGuard<double> { static constexpr double value = 9.094947017729282e-13; };  // 2^-40
-enum : int32_t { PH_NEED = 0, PH_ENTER = 1, PH_FLIGHT = 2, PH_DONE = 3 };
+enum : int32_t { PH_NEED = 0, PH_ENTER = 1, PH_FLIGHT = 2, PH_DONE = 3, PH_SPHERE = 4, PH_END = 5 };
 
 // Constants of one layer, derived on the host in FP64.
 template <typename R>
@@ -118,6 +123,7 @@ struct Args {
   uint32_t* chunk_counter;
   unsigned long long* slots; // [n_points][kSlotWords]
   double* per_packet;        // optional [n][6]
+  uint32_t idle_lanes;       // lanes waiting for a rare step that trigger a rare pass
 };
 
 // ---------------------------------------------------------------------------
@@ -271,6 +277,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
 
   const Point<R>* gpts = reinterpret_cast<const Point<R>*>(a.points);
   const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+  const uint32_t idle_thr = a.idle_lanes;
 
   // wave-uniform queue state
   int cur_set = 1;
@@ -281,8 +288,10 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
   bool exhausted = false;
 
   // lane state
-  int phase = PH_NEED, side = 0, layer = 0, set = 0;
+  int phase = PH_NEED, side = 0, layer = 0, set = 0, term = 0;
+  bool aborted = false;
   R w = 0, z = 0, muz = 1, s = 0, A = 0;
+  R su1 = 0, su2 = 0;  // return draw and new |mu_z| of a pending sphere decision
   uint64_t pid = 0, outidx = 0;
   uint32_t ev = 0, stream = 0;
   uint32_t e_int = 0, e_bnd = 0, e_inc = 0, e_sph = 0, e_rei = 0;
@@ -292,77 +301,140 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
   int L_kind = KIND_GLASS;
 
   for (;;) {
-    // ---------------- refill idle lanes from the wave's window ----------------
-    uint64_t need = __ballot(phase == PH_NEED);
-    while (need) {
-      if (win_next == win_end) {
-        if (exhausted) {
-          if (phase == PH_NEED) phase = PH_DONE;
-          break;
+    // Lanes in flight run their next event every iteration.  The rare steps
+    // (sphere decision, end-of-packet tally, refill, entrance) run in a
+    // "rare pass" only once idle_thr lanes wait for one (or none is in
+    // flight), so the wave does not pay for their code on every iteration.
+    // Every packet still draws the same Philox block for the same event.
+    const uint64_t fly = __ballot(phase == PH_FLIGHT);
+    const uint64_t idle = __ballot(phase != PH_FLIGHT && phase != PH_DONE);
+    if ((fly | idle) == 0) break;
+    const bool rare = idle != 0 && (fly == 0 || (uint32_t)__popcll(idle) >= idle_thr);
+
+    if (rare) {
+      // ---- sphere capture / return (CUDAISMLMC.h:377-410, 442-475, 518-550, 583-615)
+      if (phase == PH_SPHERE) {
+        const Point<R>& P = pts[set];
+        e_sph += 1;
+        if (DM<R>::abs(muz) >= P.cos_spec[side]) {
+          term = side ? 4 : 2;
+          phase = PH_END;
+        } else if (su1 < P.p_return[side]) {
+          muz = side ? -su2 : su2;
+          phase = PH_ENTER;
+          e_rei += 1;
+          if (ev > kMaxEventsPerPacket) { phase = PH_END; aborted = true; }
+        } else {
+          term = side ? 3 : 1;
+          phase = PH_END;
         }
-        const int ns = cur_set ^ 1;
-        const uint32_t busy = ns ? outstanding1 : outstanding0;
-        if (busy) break;  // lanes of the chunk held in that slot still run
-        const uint32_t sp = ns ? set_point1 : set_point0;
-        if (sp != 0xffffffffu) flush_slot<R>(ns ? acc1 : acc0, a.slots, sp, lane);
-        uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(a.chunk_counter, 1u);
-        k = __builtin_amdgcn_readfirstlane(k);
-        if (k >= a.n_chunks) {
-          exhausted = true;
-          if (ns) set_point1 = 0xffffffffu; else set_point0 = 0xffffffffu;
-          continue;
+      }
+      // ---- end of packet: tallies into the chunk slot's LDS accumulator
+      const bool ending = (phase == PH_END);
+      if (ending) {
+        unsigned long long* acc = set ? acc1 : acc0;
+        if (aborted) {
+          atomicAdd(&acc[15], 1ull);
+          A = (R)0;
+          term = 0;
+        } else {
+          atomicAdd(&acc[0], DM<R>::fx(A));
+          atomicAdd(&acc[5], DM<R>::fx(A * A));
+          if (term) {
+            atomicAdd(&acc[term], DM<R>::fx(w));
+            atomicAdd(&acc[5 + term], DM<R>::fx(w * w));
+          }
         }
-        const Chunk ch = a.chunks[k];
-        copy_point<R>(&pts[ns], &gpts[ch.point], lane);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (ns) set_point1 = ch.point; else set_point0 = ch.point;
-        cur_set = ns;
-        win_next = ch.first_pid;
-        win_end = ch.first_pid + ch.count;
-        win_first = ch.first_pid;
-        win_outbase = ch.out_base;
-        win_stream = ch.stream;
+        atomicAdd(&acc[10], (unsigned long long)e_int);
+        atomicAdd(&acc[11], (unsigned long long)e_bnd);
+        atomicAdd(&acc[12], (unsigned long long)e_inc);
+        atomicAdd(&acc[13], (unsigned long long)e_sph);
+        atomicAdd(&acc[14], (unsigned long long)e_rei);
+        if (kPerPacket) {
+          double* o = a.per_packet + outidx * 6;
+          o[0] = (double)A;
+          o[1] = term == 1 ? (double)w : 0.0;
+          o[2] = term == 2 ? (double)w : 0.0;
+          o[3] = term == 3 ? (double)w : 0.0;
+          o[4] = term == 4 ? (double)w : 0.0;
+          o[5] = (double)ev;
+        }
+        e_int = e_bnd = e_inc = e_sph = e_rei = 0;
+        term = 0;
+        aborted = false;
+        phase = PH_NEED;
       }
-      const uint64_t avail = win_end - win_next;
-      const uint32_t nneed = (uint32_t)__popcll(need);
-      const uint32_t take = avail < nneed ? (uint32_t)avail : nneed;
-      const uint32_t rank = lane_rank(need);
-      if (phase == PH_NEED && rank < take) {
-        pid = win_next + rank;
-        set = cur_set;
-        stream = win_stream;
-        outidx = win_outbase + (pid - win_first);
-        w = (R)1;
-        muz = pts[set].mu0_z;
-        side = 0;
-        A = (R)0;
-        ev = 0;
-        phase = PH_ENTER;
+      const uint64_t end0 = __ballot(ending && set == 0);
+      const uint64_t end1 = __ballot(ending && set == 1);
+      outstanding0 -= (uint32_t)__popcll(end0);
+      outstanding1 -= (uint32_t)__popcll(end1);
+
+      // ---- refill idle lanes from the wave's window
+      uint64_t need = __ballot(phase == PH_NEED);
+      while (need) {
+        if (win_next == win_end) {
+          if (exhausted) {
+            if (phase == PH_NEED) phase = PH_DONE;
+            break;
+          }
+          const int ns = cur_set ^ 1;
+          const uint32_t busy = ns ? outstanding1 : outstanding0;
+          if (busy) break;  // lanes of the chunk held in that slot still run
+          const uint32_t sp = ns ? set_point1 : set_point0;
+          if (sp != 0xffffffffu) flush_slot<R>(ns ? acc1 : acc0, a.slots, sp, lane);
+          uint32_t k = 0;
+          if (lane == 0) k = atomicAdd(a.chunk_counter, 1u);
+          k = __builtin_amdgcn_readfirstlane(k);
+          if (k >= a.n_chunks) {
+            exhausted = true;
+            if (ns) set_point1 = 0xffffffffu; else set_point0 = 0xffffffffu;
+            continue;
+          }
+          const Chunk ch = a.chunks[k];
+          copy_point<R>(&pts[ns], &gpts[ch.point], lane);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          if (ns) set_point1 = ch.point; else set_point0 = ch.point;
+          cur_set = ns;
+          win_next = ch.first_pid;
+          win_end = ch.first_pid + ch.count;
+          win_first = ch.first_pid;
+          win_outbase = ch.out_base;
+          win_stream = ch.stream;
+        }
+        const uint64_t avail = win_end - win_next;
+        const uint32_t nneed = (uint32_t)__popcll(need);
+        const uint32_t take = avail < nneed ? (uint32_t)avail : nneed;
+        const uint32_t rank = lane_rank(need);
+        if (phase == PH_NEED && rank < take) {
+          pid = win_next + rank;
+          set = cur_set;
+          stream = win_stream;
+          outidx = win_outbase + (pid - win_first);
+          w = (R)1;
+          muz = pts[set].mu0_z;
+          side = 0;
+          A = (R)0;
+          ev = 0;
+          phase = PH_ENTER;
+        }
+        win_next += take;
+        if (cur_set) outstanding1 += take; else outstanding0 += take;
+        need = __ballot(phase == PH_NEED);
       }
-      win_next += take;
-      if (cur_set) outstanding1 += take; else outstanding0 += take;
-      need = __ballot(phase == PH_NEED);
-    }
-    if (__ballot(phase == PH_ENTER || phase == PH_FLIGHT) == 0) {
-      if (__ballot(phase != PH_DONE) == 0) break;
-      continue;
     }
 
-    // ---------------- one event per active lane ----------------
-    const bool active = (phase == PH_ENTER || phase == PH_FLIGHT);
+    // ---------------- one event per lane in flight (and, in a rare pass, per entering lane)
+    const bool run_flight = (phase == PH_FLIGHT);
+    const bool run_enter = rare && (phase == PH_ENTER);
+    if (__ballot(run_flight || run_enter) == 0) continue;
     uint32_t rw[4];
     philox((uint32_t)pid, (uint32_t)(pid >> 32), ev, stream, k0, k1, rw);
     const R u0 = u01<R>(rw[0]), u1 = u01<R>(rw[1]), u2 = u01<R>(rw[2]), u3 = u01<R>(rw[3]);
-    ev += 1;
     const Point<R>& P = pts[set];
 
-    bool to_sphere = false, ended = false, aborted = false;
-    int term = 0;  // tally index of the exit weight (0 = none)
-
-    if (phase == PH_FLIGHT) {
+    if (run_flight) {
       const R zb = muz > (R)0 ? L_zhi : L_zlo;
       const R dz = zb - z;
       const bool glass = (L_kind == KIND_GLASS);
@@ -383,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
           muz = spin_z<R>(muz, ct, u2);
           s = DM<R>::nlog2(u0) * L_step;
         } else {
-          ended = true;
+          phase = PH_END;
         }
       } else {
         // move to the interface (CUDAISMLMC.h:180-257)
@@ -403,7 +475,9 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
           muz = DM<R>::csign(ct, muz);
           if (outside) {
             side = nxt < 0 ? 0 : 1;
-            to_sphere = true;
+            su1 = u1;
+            su2 = u2;
+            phase = PH_SPHERE;
           } else {
             const Layer<R>& NL = P.layer[nxt];
             if (NL.kind == KIND_GLASS) muz = DM<R>::csign(glass_entry_cos<R>(ct, L_n, NL.n), muz);
@@ -415,13 +489,15 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
           }
         }
       }
-    } else if (phase == PH_ENTER) {
+    } else if (run_enter) {
       // (re-)entrance through the outer face (CUDAISMLMC.h:367-369, 502-503)
       e_inc += 1;
       R ct;
       const R Re = entrance_R<R>(P, side, muz, ct);
       if (u0 < Re) {
-        to_sphere = true;
+        su1 = u1;
+        su2 = u2;
+        phase = PH_SPHERE;
       } else {
         muz = DM<R>::csign(ct, muz);
         layer = P.first_layer[side];
@@ -435,61 +511,16 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
         phase = PH_FLIGHT;
       }
     }
-    if (to_sphere) {
-      // sphere capture / return (CUDAISMLMC.h:377-410, 442-475, 518-550, 583-615)
-      e_sph += 1;
-      if (DM<R>::abs(muz) >= P.cos_spec[side]) {
-        term = side ? 4 : 2;
-        ended = true;
-      } else if (u1 < P.p_return[side]) {
-        muz = side ? -u2 : u2;
-        phase = PH_ENTER;
-        e_rei += 1;
-      } else {
-        term = side ? 3 : 1;
-        ended = true;
+    if (run_flight || run_enter) {
+      ev += 1;
+      // a packet still running after kMaxEventsPerPacket events is dropped
+      // (counted, not tallied); one waiting for its sphere decision is
+      // checked after it, once it is known whether it continues
+      if (phase != PH_END && phase != PH_SPHERE && ev > kMaxEventsPerPacket) {
+        phase = PH_END;
+        aborted = true;
       }
     }
-    if (active && !ended && ev > kMaxEventsPerPacket) {
-      ended = true;
-      aborted = true;
-    }
-
-    if (ended) {
-      unsigned long long* acc = set ? acc1 : acc0;
-      if (aborted) {
-        atomicAdd(&acc[15], 1ull);
-        A = (R)0;
-        term = 0;
-      } else {
-        atomicAdd(&acc[0], DM<R>::fx(A));
-        atomicAdd(&acc[5], DM<R>::fx(A * A));
-        if (term) {
-          atomicAdd(&acc[term], DM<R>::fx(w));
-          atomicAdd(&acc[5 + term], DM<R>::fx(w * w));
-        }
-      }
-      atomicAdd(&acc[10], (unsigned long long)e_int);
-      atomicAdd(&acc[11], (unsigned long long)e_bnd);
-      atomicAdd(&acc[12], (unsigned long long)e_inc);
-      atomicAdd(&acc[13], (unsigned long long)e_sph);
-      atomicAdd(&acc[14], (unsigned long long)e_rei);
-      if (kPerPacket) {
-        double* o = a.per_packet + outidx * 6;
-        o[0] = (double)A;
-        o[1] = term == 1 ? (double)w : 0.0;
-        o[2] = term == 2 ? (double)w : 0.0;
-        o[3] = term == 3 ? (double)w : 0.0;
-        o[4] = term == 4 ? (double)w : 0.0;
-        o[5] = (double)ev;
-      }
-      e_int = e_bnd = e_inc = e_sph = e_rei = 0;
-      phase = PH_NEED;
-    }
-    const uint64_t end0 = __ballot(ended && set == 0);
-    const uint64_t end1 = __ballot(ended && set == 1);
-    outstanding0 -= (uint32_t)__popcll(end0);
-    outstanding1 -= (uint32_t)__popcll(end1);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -572,6 +603,7 @@ struct ismc_ctx {
   int64_t last_chunks = 0;
   int blocks_per_cu = 8;
   int chunk_packets = 4096;
+  int idle_lanes = 8;
   // captured launch sequences, replayed when a call repeats a shape
   struct Graph {
     std::vector<uint64_t> key;
@@ -798,6 +830,8 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   a.chunk_counter = (uint32_t*)c->counter.p;
   a.slots = (unsigned long long*)c->slots.p;
   a.per_packet = per_packet_host ? (double*)c->per_packet.p : nullptr;
+  a.idle_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->idle_lanes > 0) ? tune->idle_lanes
+                                                                                 : c->idle_lanes));
 
   // persistent grid: as many blocks as stay resident (occupancy), unless the
   // caller asks for a given number per CU

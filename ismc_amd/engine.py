@@ -128,14 +128,22 @@ class Engine:
     def simulate_batch(self, cfgs: Sequence[SampleConfig], n_packets, seed: int = 1234,
                        first_packet_ids=None, streams=None, fp64: bool = False,
                        cuda_roulette: bool = False, blocks_per_cu: int = 0,
-                       chunk_packets: int = 0, use_graph: bool = False):
+                       chunk_packets: int = 0, use_graph: bool = False, autotune: bool = False,
+                       idle_lanes: int = 0):
         """Many configurations, uneven packet counts, one persistent launch.
         use_graph replays a captured hipGraph of the launch sequence when the
         call repeats an earlier one's shape and seed (the uploads inside the
-        graph read the new configurations from pinned staging)."""
+        graph read the new configurations from pinned staging).  autotune
+        takes blocks_per_cu and chunk_packets from this engine's AutoTuner
+        (timed on first use per workload class; tallies are identical for
+        every launch shape)."""
         npts = len(cfgs)
         if npts == 0:
             return []
+        if autotune:
+            kw = self.tuner.params(cfgs, n_packets, fp64=fp64, seed=seed)
+            blocks_per_cu, chunk_packets = kw["blocks_per_cu"], kw["chunk_packets"]
+            idle_lanes = kw.get("idle_lanes", idle_lanes)
         arr = (_lib.Config * npts)(*[c.to_c() for c in cfgs])
         npk = np.ascontiguousarray(np.broadcast_to(np.asarray(n_packets, dtype=np.uint64), (npts,)))
         first = None
@@ -144,7 +152,7 @@ class Engine:
         st = None
         if streams is not None:
             st = np.ascontiguousarray(np.broadcast_to(np.asarray(streams, dtype=np.uint32), (npts,)))
-        tune = _lib.Tune(int(blocks_per_cu), int(chunk_packets), int(bool(use_graph)), 0)
+        tune = _lib.Tune(int(blocks_per_cu), int(chunk_packets), int(bool(use_graph)), int(idle_lanes))
         outs = (_lib.Tally * npts)()
         P64 = ctypes.POINTER(ctypes.c_uint64)
         P32 = ctypes.POINTER(ctypes.c_uint32)
@@ -156,6 +164,13 @@ class Engine:
         _lib.check(self._ctx, rc, "ismc_simulate_batch")
         ms = self.last_launch()["kernel_ms"]
         return [_result(outs[i], ms) for i in range(npts)]
+
+    @property
+    def tuner(self):
+        if getattr(self, "_tuner", None) is None:
+            from .autotune import AutoTuner
+            self._tuner = AutoTuner(self)
+        return self._tuner
 
     def graph_stats(self):
         cap = ctypes.c_int64()

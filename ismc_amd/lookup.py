@@ -212,7 +212,7 @@ def run_lookup(ini, samples=None, seed=1234, engine: Optional[Engine] = None, rn
 
 def adaptive_sweep(cfgs: List[SampleConfig], rse=0.005, pilot=10_000, n_min=10_000,
                    n_max=1_000_000, seed=1234, engine: Optional[Engine] = None,
-                   chunk_packets=0, blocks_per_cu=0):
+                   chunk_packets=0, blocks_per_cu=0, autotune=False):
     """Uneven packet counts per configuration: a pilot of `pilot` packets
     estimates total reflectance R = R_d + R_s and its per-packet variance
     (a packet leaves its weight in exactly one exit tally, so
@@ -223,7 +223,7 @@ def adaptive_sweep(cfgs: List[SampleConfig], rse=0.005, pilot=10_000, n_min=10_0
     npts = len(cfgs)
     streams = np.arange(npts, dtype=np.uint32)
     p = eng.simulate_batch(cfgs, pilot, seed=seed, streams=streams, chunk_packets=chunk_packets,
-                           blocks_per_cu=blocks_per_cu)
+                           blocks_per_cu=blocks_per_cu, autotune=autotune)
     need = np.zeros(npts, dtype=np.int64)
     for k, r in enumerate(p):
         R = r.mean["R_d"] + r.mean["R_s"]
@@ -233,7 +233,8 @@ def adaptive_sweep(cfgs: List[SampleConfig], rse=0.005, pilot=10_000, n_min=10_0
     extra = np.maximum(need - pilot, 0)
     m = eng.simulate_batch(cfgs, extra.astype(np.uint64), seed=seed, streams=streams,
                            first_packet_ids=np.full(npts, pilot, dtype=np.uint64),
-                           chunk_packets=chunk_packets, blocks_per_cu=blocks_per_cu) \
+                           chunk_packets=chunk_packets, blocks_per_cu=blocks_per_cu,
+                           autotune=autotune) \
         if extra.sum() > 0 else None
     out = []
     for k in range(npts):

@@ -227,6 +227,16 @@ def init_params(arch, seed=0):
     return np.concatenate(parts).astype(np.float32)
 
 
+def packaged_weights():
+    """Path of the trained weights shipped with the package (2^20 MC-labelled
+    samples at 1e5 packets each)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate.npz")
+    if not os.path.exists(path):
+        raise FileNotFoundError("no packaged surrogate weights at %s; train them with "
+                                "python -m ismc_amd.train_surrogate --out %s" % (path, path))
+    return path
+
+
 def train_surrogate(inputs, tallies, epochs=60, batch=4096, lr=2e-3, seed=0, holdout=0.05,
                     device=0, lr_decay_every=20):
     """Fit the MLP to MC tallies (MSE on logits); returns (Surrogate,

@@ -83,3 +83,22 @@ def test_run_lookup_from_ini(engine, tmp_path):
     pr, pt = lookup.write_tables(res, str(tmp_path))
     m, s = lookup.read_csv_table(pr, 3)
     assert np.allclose(m, res.R_d, rtol=1e-5) and np.allclose(s, res.R_d_unc, rtol=1e-5, atol=1e-12)
+
+
+def test_autotuned_launch_is_bitwise_identical(engine):
+    from ismc_amd.autotune import AutoTuner
+    cfgs = [single_layer(a, s) for a in (0.05, 1.0, 8.0) for s in (5.0, 50.0, 400.0)]
+    npk = np.array([20000, 3000, 90000, 1000, 50000, 7000, 40000, 2000, 60000], dtype=np.uint64)
+    base = engine.simulate_batch(cfgs, npk, seed=77)
+    tuner = AutoTuner(engine, budget_packets=200_000, chunks=(512, 4096, 65536), blocks=(0, 1),
+                      idle=(4, 8), repeats=1)
+    kw = tuner.params(cfgs, npk)
+    assert kw["chunk_packets"] in (512, 4096, 65536) and kw["blocks_per_cu"] in (0, 1)
+    assert kw["idle_lanes"] in (4, 8)
+    assert len(tuner.report(cfgs, npk)) == 5
+    for b, c, i in ((0, 512, 1), (1, 65536, 64), (1, 512, 16), (0, 65536, 5)):
+        r = engine.simulate_batch(cfgs, npk, seed=77, blocks_per_cu=b, chunk_packets=c, idle_lanes=i)
+        for x, y in zip(r, base):
+            assert x.sum == y.sum and x.sumsq == y.sumsq and x.events == y.events
+    r = engine.simulate_batch(cfgs, npk, seed=77, autotune=True)
+    assert all(x.sum == y.sum for x, y in zip(r, base))
