@@ -1,0 +1,120 @@
+"""Summaries of rocprofv3 output for the transport kernel.
+
+Reads the CSV directories written by
+
+    rocprofv3 --kernel-trace --stats --output-format csv -d KT -o kt -- python3 bench.py ...
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d F -o f -- python3 bench.py ... --copy-probe BYTES
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d W -o w -- python3 bench.py ... --copy-probe BYTES
+    rocprofv3 --pmc <SQ/GRBM activity counters> --output-format csv -d A -o a -- python3 bench.py ...
+
+and prints one JSON object: the kernel's average duration from the trace,
+HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (each scaled by what the
+same counter reads for the plain copy kernel of known size), and the
+activity counters per packet.
+
+    python -m ismc_amd.profiling --packets 268435456 --copy-bytes 1073741824 \\
+        --trace KT --fetch F --write W --pmc A --pmc B > counters.json
+"""
+import argparse
+import csv
+import glob
+import json
+import os
+import sys
+
+KERNEL = "transport_kernel<float, false>"
+COPY = "copy_probe_kernel"
+
+
+def _csv(directory, suffix):
+    hits = sorted(glob.glob(os.path.join(directory, "*" + suffix)))
+    if not hits:
+        raise FileNotFoundError("no *%s under %s" % (suffix, directory))
+    with open(hits[0], newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def counters(directory, kernel):
+    """Average value per dispatch of every counter collected for `kernel`."""
+    per = {}
+    for row in _csv(directory, "counter_collection.csv"):
+        if kernel not in row["Kernel_Name"]:
+            continue
+        d = per.setdefault(row["Counter_Name"], {})
+        d[row["Dispatch_Id"]] = d.get(row["Dispatch_Id"], 0.0) + float(row["Counter_Value"])
+    return {k: sum(v.values()) / len(v) for k, v in per.items() if v}
+
+
+def dispatch_info(directory, kernel):
+    for row in _csv(directory, "counter_collection.csv"):
+        if kernel in row["Kernel_Name"]:
+            return {"grid": int(row["Grid_Size"]), "workgroup": int(row["Workgroup_Size"]),
+                    "vgpr": int(row["VGPR_Count"]), "sgpr": int(row["SGPR_Count"]),
+                    "lds_bytes": int(row["LDS_Block_Size"])}
+    return {}
+
+
+def kernel_stats(directory):
+    return [{"name": r["Name"], "calls": int(r["Calls"]), "avg_ns": float(r["AverageNs"]),
+             "min_ns": float(r["MinNs"]), "max_ns": float(r["MaxNs"]), "percent": float(r["Percentage"])}
+            for r in _csv(directory, "kernel_stats.csv")]
+
+
+def summarize(packets, copy_bytes, trace=None, fetch=None, write=None, pmc=(), kernel=KERNEL):
+    out = {"kernel": kernel, "packets": packets}
+    if trace:
+        st = kernel_stats(trace)
+        out["trace"] = st
+        k = [s for s in st if kernel in s["name"]]
+        if k:
+            out["avg_ns"] = k[0]["avg_ns"]
+            out["packets_per_s"] = packets / (k[0]["avg_ns"] * 1e-9)
+    if fetch and write:
+        f_copy = counters(fetch, COPY)["FETCH_SIZE"]
+        w_copy = counters(write, COPY)["WRITE_SIZE"]
+        fs = copy_bytes / 1024.0 / f_copy
+        ws = copy_bytes / 1024.0 / w_copy
+        fk = counters(fetch, kernel)["FETCH_SIZE"]
+        wk = counters(write, kernel)["WRITE_SIZE"]
+        out.update({"fetch_scale": fs, "write_scale": ws,
+                    "hbm_fetch_bytes": fk * 1024.0 * fs, "hbm_write_bytes": wk * 1024.0 * ws})
+        out["hbm_bytes_per_packet"] = (out["hbm_fetch_bytes"] + out["hbm_write_bytes"]) / packets
+        out.setdefault("raw", {}).update({"FETCH_SIZE": fk, "WRITE_SIZE": wk})
+    raw = out.setdefault("raw", {})
+    for d in pmc:
+        raw.update(counters(d, kernel))
+        out.setdefault("dispatch", dispatch_info(d, kernel))
+    per = lambda k: raw[k] / packets if k in raw else None
+    out["per_packet"] = {
+        "valu_wave_insts": per("SQ_INSTS_VALU"),
+        "trans_f32_wave_insts": per("SQ_INSTS_VALU_TRANS_F32"),
+        "int64_wave_insts": per("SQ_INSTS_VALU_INT64"),
+        "salu_insts": per("SQ_INSTS_SALU"),
+        "lds_insts": per("SQ_INSTS_LDS"),
+    }
+    if "SQ_THREAD_CYCLES_VALU" in raw and "SQ_ACTIVE_INST_VALU" in raw:
+        lanes = raw["SQ_THREAD_CYCLES_VALU"] / raw["SQ_ACTIVE_INST_VALU"]
+        out["active_lanes_per_valu_cycle"] = lanes
+        out["simd_lane_efficiency"] = lanes / 64.0
+    if "SQ_WAVES" in raw:
+        out["waves"] = raw["SQ_WAVES"]
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--packets", type=int, required=True, help="packets per profiled launch")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--trace")
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--pmc", action="append", default=[])
+    ap.add_argument("--kernel", default=KERNEL)
+    a = ap.parse_args(argv)
+    json.dump(summarize(a.packets, a.copy_bytes, a.trace, a.fetch, a.write, a.pmc, a.kernel),
+              sys.stdout, indent=1)
+    print()
+
+
+if __name__ == "__main__":
+    main()
