@@ -152,13 +152,24 @@ __global__ void reduce_slabs(const float* slabs, int k_split, size_t n, float* o
   out[i] = s;
 }
 
-// db[n] = sum_b dZ[b][n] (bf16 input), one thread per column, fixed order
-__global__ void bias_grad(const bf16_t* dZ, int B, int N, float* db) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// Partial bias gradients: partial[slab][n] = sum of dZ[b][n] (bf16) over the
+// rows of one slab.  A block covers 64 columns (one row of 64 bf16 is a
+// 128-byte coalesced load) with 4 row phases; the phases are combined in a
+// fixed order, and reduce_slabs sums the slabs in a fixed order after it,
+// so the result is deterministic.
+constexpr int kBiasSlabs = 16;
+__global__ void bias_partial(const bf16_t* dZ, int B, int N, int rows_per_slab, float* partial) {
+  __shared__ float red[4][64];
+  const int c = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + c;
+  const int r0 = blockIdx.y * rows_per_slab;
+  const int r1 = min(B, r0 + rows_per_slab);
   float s = 0.f;
-  for (int b = 0; b < B; ++b) s += bf2f(dZ[(size_t)b * N + n]);
-  db[n] = s;
+  if (n < N)
+    for (int b = r0 + q; b < r1; b += 4) s += bf2f(dZ[(size_t)b * N + n]);
+  red[q][c] = s;
+  __syncthreads();
+  if (q == 0 && n < N) partial[(size_t)blockIdx.y * N + n] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
 }
 
 // MSE on the first n_out columns: dY = 2 (Y - T) / (B n_out), loss partials per block
@@ -353,7 +364,16 @@ static int backward(ismc_mlp* p, int B, float* loss_host, float* loss_dev = null
       return rc;
     const size_t nw = (size_t)Ly.K * Ly.N;
     reduce_slabs<<<(unsigned)((nw + 255) / 256), 256, 0, p->stream>>>(p->slabs, ks, nw, p->g + Ly.w_off);
-    bias_grad<<<(Ly.N + 255) / 256, 256, 0, p->stream>>>(dcur, B, Ly.N, p->g + Ly.b_off);
+    // bias gradient: column sums over kBiasSlabs row slabs (the split-K slabs
+    // above are consumed by now and hold the partials), then a fixed-order sum
+    {
+      const int rows = (B + kBiasSlabs - 1) / kBiasSlabs;
+      const int nslab = (B + rows - 1) / rows;
+      bias_partial<<<dim3((unsigned)((Ly.N + 63) / 64), (unsigned)nslab), 256, 0, p->stream>>>(
+          dcur, B, Ly.N, rows, p->slabs);
+      reduce_slabs<<<(unsigned)((Ly.N + 255) / 256), 256, 0, p->stream>>>(p->slabs, nslab, (size_t)Ly.N,
+                                                                           p->g + Ly.b_off);
+    }
     MCHECK(p, hipGetLastError());
     if (l > 0) {
       // dZ_prev = (dZ W^T) * silu'(Z_prev)
