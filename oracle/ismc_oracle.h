@@ -28,7 +28,7 @@
  *   philox      Random123 known-answer vectors + rocRAND's own Philox rounds
  *   fresnel     textbook closed forms (normal incidence, TIR, Brewster,
  *               reciprocity, grazing); snell: unit norm + Snell's invariant
- *   entrance    closed-form glass-entrance reflectance (configs[0]: 0.0441282)
+ *   entrance    closed-form glass-entrance reflectance (0.0441282 for air/1.52 glass/1.40 sample at normal incidence)
  *   HG spin     E[cos]=g, E[P2]=g^2, KS vs HG CDF, |mu|=1
  *   sphere      closed-form return probability, black-sphere == no-sphere
  *   transport   absorption-only slab closed forms, matched-boundary ballistic

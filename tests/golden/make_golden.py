@@ -49,11 +49,11 @@ def _layer(n, g, t, mu_a, mu_s):
 #   "ismlmc" = ISMLMC.ISMLMC (multilayer, MLMC.MC, roulette 1e-4)
 #   "ismc"   = ISMC.ISMC (single slab mirror, roulette 1e-3)
 CASES = {
-    # configs[0]: slab between two glass slides, dual spheres
+    # slab between two glass slides, dual spheres
     "cfg0_slides_dual": ("ismlmc", {
         "layers": [GLASS, _layer(1.40, 0.9, 0.1, 1.0, 100.0), GLASS],
         "R": SPH0, "T": SPH0, "specular_included": 0}, 100000, 10),
-    # configs[1]: three turbid layers, dual spheres
+    # three turbid layers, dual spheres
     "cfg1_three_layer_dual": ("ismlmc", {
         "layers": [_layer(1.37, 0.8, 0.01, 0.5, 150.0), _layer(1.40, 0.9, 0.1, 2.0, 80.0),
                    _layer(1.44, 0.85, 0.2, 0.2, 120.0)],
@@ -64,11 +64,11 @@ CASES = {
         "R": {"present": 1, "pw": 0.99, "fs": 0.02351, "fp": 0.054638, "fd": 0.02351, "angle": math.pi / 10},
         "T": {"present": 0, "pw": 0.99, "fs": 0.054638, "fp": 0.02351, "fd": 0.02351, "angle": 0.22 * math.pi / 180},
         "specular_included": 0, "roulette_threshold": 1e-3}, 50000, 10),
-    # configs[0] sample without slides through ISMC.py, dual spheres
+    # the slab-between-slides sample without slides, through ISMC.py, dual spheres
     "cfg0_sample_ismc_dual": ("ismc", {
         "layers": [_layer(1.40, 0.9, 0.1, 1.0, 100.0)],
         "R": SPH0, "T": SPH0, "specular_included": 0, "roulette_threshold": 1e-3}, 50000, 10),
-    # configs[0] with the 8.1 degree beam (specular_included, Q18)
+    # slab between slides with the 8.1 degree beam (specular_included)
     "cfg0_slides_specular": ("ismlmc", {
         "layers": [GLASS, _layer(1.40, 0.9, 0.1, 1.0, 100.0), GLASS],
         "R": SPH0, "T": SPH0, "specular_included": 1}, 20000, 10),
@@ -88,7 +88,7 @@ CASES = {
         "R": {"present": 0, "pw": 0.99, "fs": 0.02351, "fp": 0.054638, "fd": 0.02351, "angle": 0.0},
         "T": {"present": 1, "pw": 0.99, "fs": 0.054638, "fp": 0.02351, "fd": 0.02351, "angle": 0.0},
         "specular_included": 0, "roulette_threshold": 1e-3}, 20000, 10),
-    # configs[2] grid corners/centre (single layer, g=0.8, n=1.4, t=0.1, dual spheres), ISMC.py
+    # 64x64 lookup-grid corners/centre (single layer, g=0.8, n=1.4, t=0.1, dual spheres), ISMC.py
     "grid_mua0.01_mus1": ("ismc", {
         "layers": [_layer(1.40, 0.8, 0.1, 0.01, 1.0)],
         "R": SPH0, "T": SPH0, "specular_included": 0, "roulette_threshold": 1e-3}, 20000, 10),
