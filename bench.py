@@ -58,6 +58,51 @@ def cpu_baseline(seconds):
                          "sample": "%d packets, C oracle FP64, OpenMP" % m}}
 
 
+def cpu_inverse_baseline(packets_per_eval=2000, target_packets=1_000_000):
+    """The reference's CPU inverse path: scipy's Powell (minimization.py:264-266)
+    minimizing the relative R_d/T_d cost of the pure-Python restatement, on
+    1 core.  Bounded sample: one solve for (mu_a 1 /cm, mu_s' 10 /cm; g 0.8,
+    n 1.40, t 0.1 cm, dual spheres) at `packets_per_eval` packets per cost
+    evaluation (fixed seed per evaluation), from the plain solver's generic
+    guess in log coordinates; solves/s is extrapolated to `target_packets`
+    packets per evaluation (the plain GPU solver's N) from the measured
+    Python packets/s and the solve's evaluation count."""
+    import math
+    import numpy as np
+    from scipy.optimize import minimize
+    from ismc_amd import inverse
+    from ismc_amd.minimization import cost_from
+    from oracle import ismc_py, oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import spec_of
+    probe = inverse.problem(0.3, 0.5)
+    truth = oracle.simulate(spec_of(probe.configs(1.0, 10.0)[0][0]), 1_000_000, seed=11,
+                            threads=min(len(os.sched_getaffinity(0)), 16))["mean"]
+    prob = inverse.problem(truth["R_d"], truth["T_d"])
+    stats = {"evals": 0, "packets": 0, "seconds": 0.0}
+
+    def cost(y):
+        a, s = math.exp(y[0]), math.exp(y[1])
+        t0 = time.perf_counter()
+        A, Rd, Rs, Td, Tu = ismc_py.simulate(spec_of(prob.configs(a, s)[0][0]), packets_per_eval, seed=5)
+        stats["seconds"] += time.perf_counter() - t0
+        stats["evals"] += 1
+        stats["packets"] += packets_per_eval
+        return cost_from({"R_d": Rd, "T_d": Td}, prob.measured)
+
+    t0 = time.perf_counter()
+    res = minimize(cost, np.log(inverse.GENERIC_GUESS), method="Powell", tol=3e-3)
+    wall = time.perf_counter() - t0
+    rate = stats["packets"] / stats["seconds"]
+    secs_per_solve = stats["evals"] * target_packets / rate
+    return {"value": 1.0 / secs_per_solve, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": "one scipy Powell solve on the pure-Python restatement at %d packets per "
+                      "evaluation: %d evaluations, %.1f s, %.3g packets/s; extrapolated to %d "
+                      "packets per evaluation" % (packets_per_eval, stats["evals"], wall, rate,
+                                                   target_packets),
+            "solution": [math.exp(res.x[0]), math.exp(res.x[1])]}
+
+
 def lookup_sweep(eng):
     """64 x 64 single-layer lookup grid (mu_a 0.01..10 /cm, mu_s 1..1000 /cm,
     g 0.8, n 1.40, t 0.1 cm, dual spheres), adaptive packet counts: a 1e4
@@ -90,7 +135,10 @@ def inverse_bench(eng, count):
     r = inverse.run(sur, count=count, plain_count=8, engine=eng, methods=("newton",), with_ablation=False,
                     log=lambda *x: print(*x, file=sys.stderr, flush=True))
     n, p = r["seeded_newton"], r["plain"]
+    print("bench: timing the CPU inverse baseline", file=sys.stderr, flush=True)
+    cpu = cpu_inverse_baseline()
     return {"measurements": count, "noise": "1% on R_d, T_d", "solves_per_s": n["solves_per_s"],
+            "cpu_baseline": cpu,
             "plain_solves_per_s": p["solves_per_s"], "speedup_vs_plain": r["speedup_newton_vs_plain"],
             "fits_as_well_as_truth": n["fits_as_well_as_truth"],
             "plain_fits_as_well_as_truth": p["fits_as_well_as_truth"],
