@@ -47,7 +47,8 @@ class Tally(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("blocks_per_cu", ctypes.c_int32), ("chunk_packets", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("idle_lanes", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("idle_lanes", ctypes.c_int32),
+                ("bound_lanes", ctypes.c_int32), ("_reserved", ctypes.c_int32)]
 
 
 class DeviceInfo(ctypes.Structure):

@@ -1,13 +1,14 @@
 """Launch-shape autotuner for batched transport.
 
 A batched launch is one persistent grid that pulls (configuration, packet
-range) chunks off a work queue.  Three knobs shape it: packets per queue
+range) chunks off a work queue.  Four knobs shape it: packets per queue
 chunk (small chunks balance uneven packet counts and short-lived packets;
 large chunks cut queue traffic and per-chunk tally flushes), idle lanes
 (how many lanes of a wave wait for an entrance, sphere decision or new
 packet before the wave serves them: few means the rare code runs nearly
-every step, many means lanes sit idle longer), and workgroups per CU
-(0 = as many as the occupancy calculator allows).
+every step, many means lanes sit idle longer), bound lanes (the same for
+lanes waiting at an interface), and workgroups per CU (0 = as many as the
+occupancy calculator allows).
 
 The tallies do not depend on any knob: a packet's random numbers are a
 function of (seed, stream, packet id) only, and the tallies are exact
@@ -15,7 +16,8 @@ fixed-point sums, so every launch shape returns bit-identical results.  The
 tuner therefore only measures: it runs each candidate shape on a bounded
 sample of the workload (the same configurations, packet counts scaled down
 in proportion to a packet budget), one knob at a time (chunk size, then
-idle lanes at the best chunk size, then workgroups per CU), keeps the
+idle lanes, bound lanes and workgroups per CU, each at the best values
+found so far), keeps the
 shape with the highest packets per kernel-second, and caches the choice
 per workload signature.
 """
@@ -26,6 +28,7 @@ import numpy as np
 
 CHUNKS = (512, 1024, 2048, 4096, 8192, 16384, 65536)
 IDLE = (4, 8, 16, 32)
+BOUND = (4, 8, 16, 32)
 BLOCKS = (0, 2, 4)
 
 
@@ -52,15 +55,17 @@ def sample_counts(n_packets, npts, budget) -> np.ndarray:
 
 
 class AutoTuner:
-    """Chooses (chunk_packets, idle_lanes, blocks_per_cu) for an Engine's batched launches."""
+    """Chooses (chunk_packets, idle_lanes, bound_lanes, blocks_per_cu) for an Engine's batched launches."""
 
     def __init__(self, engine, budget_packets: int = 1 << 26, chunks: Sequence[int] = CHUNKS,
-                 blocks: Sequence[int] = BLOCKS, idle: Sequence[int] = IDLE, repeats: int = 2):
+                 blocks: Sequence[int] = BLOCKS, idle: Sequence[int] = IDLE,
+                 bound: Sequence[int] = BOUND, repeats: int = 2):
         self.engine = engine
         self.budget = int(budget_packets)
         self.chunks = tuple(int(c) for c in chunks)
         self.blocks = tuple(int(b) for b in blocks)
         self.idle = tuple(int(i) for i in idle)
+        self.bound = tuple(int(b) for b in bound)
         self.repeats = int(repeats)
         self.cache: Dict[Tuple, Dict] = {}
         self.tables: Dict[Tuple, list] = {}
@@ -93,10 +98,13 @@ class AutoTuner:
             table.append(t)
             return t
 
-        # start from the library's defaults (occupancy-limited grid, 8 idle lanes)
+        # start from the library's defaults (occupancy-limited grid, 8 idle
+        # lanes, 16 bound lanes)
         best = {"blocks_per_cu": 0, "chunk_packets": chunks[0],
-                "idle_lanes": 8 if 8 in self.idle else self.idle[0]}
-        for knob, values in (("chunk_packets", chunks), ("idle_lanes", self.idle), ("blocks_per_cu", self.blocks)):
+                "idle_lanes": 8 if 8 in self.idle else self.idle[0],
+                "bound_lanes": 16 if 16 in self.bound else self.bound[0]}
+        for knob, values in (("chunk_packets", chunks), ("idle_lanes", self.idle),
+                             ("bound_lanes", self.bound), ("blocks_per_cu", self.blocks)):
             rows = [timed(dict(best, **{knob: v})) for v in values]
             best[knob] = max(rows, key=lambda t: t["packets_per_s"])[knob]
         self.tables[signature(cfgs, n_packets, fp64)] = table

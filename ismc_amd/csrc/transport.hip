@@ -11,15 +11,18 @@
 //
 //  * Event machine.  Each loop iteration advances every lane in flight by
 //    one event of its packet: a flight that ends in an interaction (hop,
-//    drop, roulette, spin) or on an interface (Fresnel reflect / refract /
-//    exit).  The rare steps (an entrance through the outer face, a sphere
-//    decision, the end-of-packet tally, taking a new packet) wait until
-//    enough lanes of the wave need one (the idle-lane threshold) or no
-//    lane is in flight, and then run together in one "rare pass".  With
-//    64 lanes nearly every iteration has some lane at a rare step, so
-//    running them at once instead of on every iteration removes most of
-//    the divergent work.  A wave never waits for its longest history:
-//    finished lanes take new packets at the next rare pass.
+//    drop, roulette, spin) or on an interface.  A lane that reaches an
+//    interface records its event's draws and waits; the interface code
+//    (Fresnel reflect / refract / exit, layer change) runs for all waiting
+//    lanes at once when enough of them wait (the bound-lane threshold).
+//    The rare steps (an entrance through the outer face, a sphere
+//    decision, the end-of-packet tally, taking a new packet) likewise wait
+//    for the idle-lane threshold and run together in one "rare pass".
+//    With 64 lanes nearly every iteration has some lane at an interface and
+//    some lane at a rare step, so running those paths for many lanes at
+//    once instead of on every iteration removes most of the divergent
+//    work.  A wave never waits for its longest history: finished lanes
+//    take new packets at the next rare pass.
 //  * Only z and mu_z are carried.  The slab is laterally infinite and
 //    every tally, boundary distance, Fresnel test and port test depends on
 //    z and mu_z alone; after a spin the new mu_z depends only on (mu_z,
@@ -80,7 +83,8 @@ enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2 };
 template <typename R> struct GlassGuard;
 template <> struct GlassGuard<float> { static constexpr float value = 9.5367431640625e-07f; };  // 2^-20
 template <> struct GlassGuard<double> { static constexpr double value = 9.094947017729282e-13; };  // 2^-40
-enum : int32_t { PH_NEED = 0, PH_ENTER = 1, PH_FLIGHT = 2, PH_DONE = 3, PH_SPHERE = 4, PH_END = 5 };
+enum : int32_t { PH_NEED = 0, PH_ENTER = 1, PH_FLIGHT = 2, PH_DONE = 3, PH_SPHERE = 4, PH_END = 5,
+                 PH_BOUND = 6 };
 
 // Constants of one layer, derived on the host in FP64.
 template <typename R>
@@ -124,6 +128,7 @@ struct Args {
   unsigned long long* slots; // [n_points][kSlotWords]
   double* per_packet;        // optional [n][6]
   uint32_t idle_lanes;       // lanes waiting for a rare step that trigger a rare pass
+  uint32_t bound_lanes;      // lanes waiting at an interface that trigger an interface pass
 };
 
 // ---------------------------------------------------------------------------
@@ -278,6 +283,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
   const Point<R>* gpts = reinterpret_cast<const Point<R>*>(a.points);
   const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
   const uint32_t idle_thr = a.idle_lanes;
+  const uint32_t bound_thr = a.bound_lanes;
 
   // wave-uniform queue state
   int cur_set = 1;
@@ -292,6 +298,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
   bool aborted = false;
   R w = 0, z = 0, muz = 1, s = 0, A = 0;
   R su1 = 0, su2 = 0;  // return draw and new |mu_z| of a pending sphere decision
+  R bu0 = 0;           // Fresnel draw of a pending interface hit
   uint64_t pid = 0, outidx = 0;
   uint32_t ev = 0, stream = 0;
   uint32_t e_int = 0, e_bnd = 0, e_inc = 0, e_sph = 0, e_rei = 0;
@@ -301,15 +308,22 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
   int L_kind = KIND_GLASS;
 
   for (;;) {
-    // Lanes in flight run their next event every iteration.  The rare steps
-    // (sphere decision, end-of-packet tally, refill, entrance) run in a
-    // "rare pass" only once idle_thr lanes wait for one (or none is in
-    // flight), so the wave does not pay for their code on every iteration.
-    // Every packet still draws the same Philox block for the same event.
+    // Lanes in flight run their next event every iteration.  A flight that
+    // reaches an interface only records its event's draws and waits; the
+    // interface code (Fresnel, refraction, layer change) runs in an
+    // "interface pass" once bound_thr lanes wait (or none is in flight),
+    // after which those lanes go on with their next flight event in the
+    // same iteration.  The rare steps (sphere decision, end-of-packet tally,
+    // refill, entrance) likewise run in a "rare pass" once idle_thr lanes
+    // wait for one.  So the wave does not pay for the code of every path on
+    // every iteration, and every packet still draws the same Philox block
+    // for the same event.
     const uint64_t fly = __ballot(phase == PH_FLIGHT);
-    const uint64_t idle = __ballot(phase != PH_FLIGHT && phase != PH_DONE);
-    if ((fly | idle) == 0) break;
-    const bool rare = idle != 0 && (fly == 0 || (uint32_t)__popcll(idle) >= idle_thr);
+    const uint64_t bnd = __ballot(phase == PH_BOUND);
+    const uint64_t idle = __ballot(phase != PH_FLIGHT && phase != PH_BOUND && phase != PH_DONE);
+    if ((fly | bnd | idle) == 0) break;
+    const bool rare = idle != 0 && ((fly | bnd) == 0 || (uint32_t)__popcll(idle) >= idle_thr);
+    const bool bpass = bnd != 0 && (fly == 0 || (uint32_t)__popcll(bnd) >= bound_thr);
 
     if (rare) {
       // ---- sphere capture / return (CUDAISMLMC.h:377-410, 442-475, 518-550, 583-615)
@@ -425,6 +439,49 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
       }
     }
 
+    const Point<R>& P = pts[set];
+    // ---------------- interface pass: the hits recorded since the last one
+    // (CUDAISMLMC.h:180-257), with the draws of the hit's own event
+    if (bpass && phase == PH_BOUND) {
+      const R zb = muz > (R)0 ? L_zhi : L_zlo;
+      const R dz = zb - z;
+      const bool glass = (L_kind == KIND_GLASS);
+      const R d = dz * DM<R>::rcp(muz);
+      z = zb;
+      s = glass ? (R)0 : s - d;
+      const int nxt = muz > (R)0 ? layer + 1 : layer - 1;
+      const bool outside = (nxt < 0) || (nxt >= P.n_layers);
+      const int nxc = outside ? layer : nxt;
+      const R nn = outside ? P.n_amb : P.layer[nxc].n;
+      R ct;
+      const R Rf = fresnel<R>(L_n, nn, muz, ct);
+      e_bnd += 1;
+      phase = PH_FLIGHT;
+      if (bu0 < Rf) {
+        muz = -muz;
+      } else {
+        muz = DM<R>::csign(ct, muz);
+        if (outside) {
+          side = nxt < 0 ? 0 : 1;
+          phase = PH_SPHERE;  // su1, su2 hold the hit's return draw and new |mu_z|
+        } else {
+          const Layer<R>& NL = P.layer[nxt];
+          if (NL.kind == KIND_GLASS) muz = DM<R>::csign(glass_entry_cos<R>(ct, L_n, NL.n), muz);
+          if (NL.kind != KIND_GLASS && s != (R)0) s *= L_mut * DM<R>::rcp(NL.mu_t);
+          layer = nxt;
+          L_zlo = NL.zlo; L_zhi = NL.zhi; L_n = NL.n; L_step = NL.step_k; L_abs = NL.absfrac;
+          L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
+          L_mut = NL.mu_t; L_kind = NL.kind;
+        }
+      }
+      // a packet still running after kMaxEventsPerPacket events is dropped
+      // (counted, not tallied)
+      if (phase == PH_FLIGHT && ev > kMaxEventsPerPacket) {
+        phase = PH_END;
+        aborted = true;
+      }
+    }
+
     // ---------------- one event per lane in flight (and, in a rare pass, per entering lane)
     const bool run_flight = (phase == PH_FLIGHT);
     const bool run_enter = rare && (phase == PH_ENTER);
@@ -432,14 +489,19 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
     uint32_t rw[4];
     philox((uint32_t)pid, (uint32_t)(pid >> 32), ev, stream, k0, k1, rw);
     const R u0 = u01<R>(rw[0]), u1 = u01<R>(rw[1]), u2 = u01<R>(rw[2]), u3 = u01<R>(rw[3]);
-    const Point<R>& P = pts[set];
 
     if (run_flight) {
       const R zb = muz > (R)0 ? L_zhi : L_zlo;
       const R dz = zb - z;
       const bool glass = (L_kind == KIND_GLASS);
       const bool hit = glass || (muz != (R)0 && s * DM<R>::abs(muz) >= DM<R>::abs(dz));
-      if (!hit) {
+      if (hit) {
+        // reaches the interface: record the event's draws for the interface pass
+        bu0 = u0;
+        su1 = u1;
+        su2 = u2;
+        phase = PH_BOUND;
+      } else {
         // hop, drop, roulette, spin (CUDAISMLMC.h:259-309)
         z += s * muz;
         const R dW = w * L_abs;
@@ -456,37 +518,6 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
           s = DM<R>::nlog2(u0) * L_step;
         } else {
           phase = PH_END;
-        }
-      } else {
-        // move to the interface (CUDAISMLMC.h:180-257)
-        const R d = dz * DM<R>::rcp(muz);
-        z = zb;
-        s = glass ? (R)0 : s - d;
-        const int nxt = muz > (R)0 ? layer + 1 : layer - 1;
-        const bool outside = (nxt < 0) || (nxt >= P.n_layers);
-        const int nxc = outside ? layer : nxt;
-        const R nn = outside ? P.n_amb : P.layer[nxc].n;
-        R ct;
-        const R Rf = fresnel<R>(L_n, nn, muz, ct);
-        e_bnd += 1;
-        if (u0 < Rf) {
-          muz = -muz;
-        } else {
-          muz = DM<R>::csign(ct, muz);
-          if (outside) {
-            side = nxt < 0 ? 0 : 1;
-            su1 = u1;
-            su2 = u2;
-            phase = PH_SPHERE;
-          } else {
-            const Layer<R>& NL = P.layer[nxt];
-            if (NL.kind == KIND_GLASS) muz = DM<R>::csign(glass_entry_cos<R>(ct, L_n, NL.n), muz);
-            if (NL.kind != KIND_GLASS && s != (R)0) s *= L_mut * DM<R>::rcp(NL.mu_t);
-            layer = nxt;
-            L_zlo = NL.zlo; L_zhi = NL.zhi; L_n = NL.n; L_step = NL.step_k; L_abs = NL.absfrac;
-            L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
-            L_mut = NL.mu_t; L_kind = NL.kind;
-          }
         }
       }
     } else if (run_enter) {
@@ -514,9 +545,10 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(Args a) {
     if (run_flight || run_enter) {
       ev += 1;
       // a packet still running after kMaxEventsPerPacket events is dropped
-      // (counted, not tallied); one waiting for its sphere decision is
-      // checked after it, once it is known whether it continues
-      if (phase != PH_END && phase != PH_SPHERE && ev > kMaxEventsPerPacket) {
+      // (counted, not tallied); one waiting at an interface or for its
+      // sphere decision is checked after it, once it is known whether it
+      // continues
+      if (phase == PH_FLIGHT && ev > kMaxEventsPerPacket) {
         phase = PH_END;
         aborted = true;
       }
@@ -604,6 +636,7 @@ struct ismc_ctx {
   int blocks_per_cu = 8;
   int chunk_packets = 4096;
   int idle_lanes = 8;
+  int bound_lanes = 16;
   // captured launch sequences, replayed when a call repeats a shape
   struct Graph {
     std::vector<uint64_t> key;
@@ -832,6 +865,8 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   a.per_packet = per_packet_host ? (double*)c->per_packet.p : nullptr;
   a.idle_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->idle_lanes > 0) ? tune->idle_lanes
                                                                                  : c->idle_lanes));
+  a.bound_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->bound_lanes > 0) ? tune->bound_lanes
+                                                                                   : c->bound_lanes));
 
   // persistent grid: as many blocks as stay resident (occupancy), unless the
   // caller asks for a given number per CU

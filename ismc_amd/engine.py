@@ -129,7 +129,7 @@ class Engine:
                        first_packet_ids=None, streams=None, fp64: bool = False,
                        cuda_roulette: bool = False, blocks_per_cu: int = 0,
                        chunk_packets: int = 0, use_graph: bool = False, autotune: bool = False,
-                       idle_lanes: int = 0):
+                       idle_lanes: int = 0, bound_lanes: int = 0):
         """Many configurations, uneven packet counts, one persistent launch.
         use_graph replays a captured hipGraph of the launch sequence when the
         call repeats an earlier one's shape and seed (the uploads inside the
@@ -144,6 +144,7 @@ class Engine:
             kw = self.tuner.params(cfgs, n_packets, fp64=fp64, seed=seed)
             blocks_per_cu, chunk_packets = kw["blocks_per_cu"], kw["chunk_packets"]
             idle_lanes = kw.get("idle_lanes", idle_lanes)
+            bound_lanes = kw.get("bound_lanes", bound_lanes)
         arr = (_lib.Config * npts)(*[c.to_c() for c in cfgs])
         npk = np.ascontiguousarray(np.broadcast_to(np.asarray(n_packets, dtype=np.uint64), (npts,)))
         first = None
@@ -152,7 +153,8 @@ class Engine:
         st = None
         if streams is not None:
             st = np.ascontiguousarray(np.broadcast_to(np.asarray(streams, dtype=np.uint32), (npts,)))
-        tune = _lib.Tune(int(blocks_per_cu), int(chunk_packets), int(bool(use_graph)), int(idle_lanes))
+        tune = _lib.Tune(int(blocks_per_cu), int(chunk_packets), int(bool(use_graph)), int(idle_lanes),
+                         int(bound_lanes), 0)
         outs = (_lib.Tally * npts)()
         P64 = ctypes.POINTER(ctypes.c_uint64)
         P32 = ctypes.POINTER(ctypes.c_uint32)

@@ -12,12 +12,13 @@ class TimedEngine:
         self.calls = []
 
     def simulate_batch(self, cfgs, n_packets, seed=0, streams=None, fp64=False, blocks_per_cu=0,
-                       chunk_packets=0, idle_lanes=0, **kw):
+                       chunk_packets=0, idle_lanes=0, bound_lanes=0, **kw):
         npk = np.broadcast_to(np.asarray(n_packets), (len(cfgs),))
         self.calls.append((blocks_per_cu, chunk_packets, int(npk.sum()), idle_lanes))
         ms = 1.0 + abs(np.log2(max(chunk_packets, 1)) - np.log2(self.best[1])) + \
             (0.0 if blocks_per_cu == self.best[0] else 0.5) + \
-            0.1 * abs(np.log2(max(idle_lanes, 1)) - np.log2(self.best[2]))
+            0.1 * abs(np.log2(max(idle_lanes, 1)) - np.log2(self.best[2])) + \
+            0.05 * abs(np.log2(max(bound_lanes, 1)) - 3)
         return [Result(n_packets=int(n), mean={}, se={}, sum={}, sumsq={}, events={}, kernel_ms=ms)
                 for n in npk]
 
@@ -27,11 +28,11 @@ def test_picks_fastest_shape_and_caches():
     t = autotune.AutoTuner(eng, budget_packets=1 << 20)
     cfgs = [workloads.single_layer(0.1 * (k + 1), 10.0) for k in range(8)]
     kw = t.params(cfgs, 1 << 22)
-    assert kw == {"blocks_per_cu": 2, "chunk_packets": 4096, "idle_lanes": 8}
+    assert kw == {"blocks_per_cu": 2, "chunk_packets": 4096, "idle_lanes": 8, "bound_lanes": 8}
     n = len(eng.calls)
     # coordinate search: each knob's values once (the shape shared by two
     # stages is timed once)
-    shapes = len(autotune.CHUNKS) + len(autotune.IDLE) + len(autotune.BLOCKS) - 2
+    shapes = len(autotune.CHUNKS) + len(autotune.IDLE) + len(autotune.BOUND) + len(autotune.BLOCKS) - 3
     assert n == 1 + shapes * t.repeats
     assert len(t.report(cfgs, 1 << 22)) == shapes
     # every timing launch stays within the budget

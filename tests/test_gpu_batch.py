@@ -91,13 +91,14 @@ def test_autotuned_launch_is_bitwise_identical(engine):
     npk = np.array([20000, 3000, 90000, 1000, 50000, 7000, 40000, 2000, 60000], dtype=np.uint64)
     base = engine.simulate_batch(cfgs, npk, seed=77)
     tuner = AutoTuner(engine, budget_packets=200_000, chunks=(512, 4096, 65536), blocks=(0, 1),
-                      idle=(4, 8), repeats=1)
+                      idle=(4, 8), bound=(8, 16), repeats=1)
     kw = tuner.params(cfgs, npk)
     assert kw["chunk_packets"] in (512, 4096, 65536) and kw["blocks_per_cu"] in (0, 1)
     assert kw["idle_lanes"] in (4, 8)
-    assert len(tuner.report(cfgs, npk)) == 5
-    for b, c, i in ((0, 512, 1), (1, 65536, 64), (1, 512, 16), (0, 65536, 5)):
-        r = engine.simulate_batch(cfgs, npk, seed=77, blocks_per_cu=b, chunk_packets=c, idle_lanes=i)
+    assert len(tuner.report(cfgs, npk)) == 6
+    for b, c, i, bl in ((0, 512, 1, 1), (1, 65536, 64, 64), (1, 512, 16, 3), (0, 65536, 5, 40)):
+        r = engine.simulate_batch(cfgs, npk, seed=77, blocks_per_cu=b, chunk_packets=c, idle_lanes=i,
+                                  bound_lanes=bl)
         for x, y in zip(r, base):
             assert x.sum == y.sum and x.sumsq == y.sumsq and x.events == y.events
     r = engine.simulate_batch(cfgs, npk, seed=77, autotune=True)
