@@ -119,11 +119,49 @@ def lookup_sweep(eng):
     wall = time.perf_counter() - t0
     kern = (info["pilot_kernel_ms"] + info["main_kernel_ms"]) / 1e3
     rse = [r["rse_R"] for r in res]
-    return {"points": len(cfgs), "total_packets": info["total_packets"], "wall_s": wall,
-            "kernel_s": kern, "packets_per_s": info["total_packets"] / wall,
-            "kernel_packets_per_s": info["total_packets"] / kern,
-            "max_rse_R": max(rse), "first_sweep_with_tuning_s": t_tune,
-            "tuned": {str(k): v for k, v in eng.tuner.cache.items()}}
+    out = {"points": len(cfgs), "total_packets": info["total_packets"], "wall_s": wall,
+           "kernel_s": kern, "packets_per_s": info["total_packets"] / wall,
+           "kernel_packets_per_s": info["total_packets"] / kern,
+           "max_rse_R": max(rse), "first_sweep_with_tuning_s": t_tune,
+           "tuned": {str(k): v for k, v in eng.tuner.cache.items()}}
+    return out, res
+
+
+def lookup_tables(sweep_res, P=1000, M=200, cpu_cells=12):
+    """Lookup tables from the 64 x 64 sweep (lookup.py:300-376): griddata onto
+    a P x P (R_d, T_d) grid on the host, then the per-cell uncertainty map
+    (M draws per cell) on the GPU.  CPU baseline: the reference's per-cell
+    loop (oracle restatement, argmin over the grid per draw) on `cpu_cells`
+    cells, extrapolated to every cell the map covers."""
+    import numpy as np
+    from ismc_amd import lookup_table as LT
+    from ismc_amd.workloads import lookup_grid
+    mua, mus, _ = lookup_grid(64, 64)
+    mus_r = mus * (1 - 0.8)
+    grid = lambda key, f: np.array([f(r, key) for r in sweep_res]).reshape(64, 64)
+    Rd, Td = grid("R_d", lambda r, k: r["mean"][k]), grid("T_d", lambda r, k: r["mean"][k])
+    Rs, Ts = grid("R_d", lambda r, k: r["se"][k]), grid("T_d", lambda r, k: r["se"][k])
+    t0 = time.perf_counter()
+    tabs = LT.make_tables(Rd, Td, Rs, Ts, mua, mus_r, P=P)
+    t_grid = time.perf_counter() - t0
+    LT.table_uncertainty(tabs, M=8, seed=1)       # first call: module load
+    t0 = time.perf_counter()
+    ua, us = LT.table_uncertainty(tabs, M=M, seed=2)
+    t_unc = time.perf_counter() - t0
+    cells = int(np.isfinite(ua).sum())
+    from oracle import table_ref
+    idx = np.argwhere(np.isfinite(ua))[:: max(1, cells // cpu_cells)][:cpu_cells]
+    t0 = time.perf_counter()
+    table_ref.uncertainty(tabs.mua, tabs.mus, tabs.rstd, tabs.tstd, tabs.Rvals, tabs.Tvals, M, 2,
+                          cells=[tuple(c) for c in idx])
+    t_cpu = (time.perf_counter() - t0) / len(idx)
+    return {"P": P, "M": M, "cells_with_uncertainty": cells, "griddata_s": t_grid,
+            "uncertainty_map_s": t_unc, "draws_per_s": cells * M / t_unc,
+            "cpu_baseline": {"value": 1.0 / (t_cpu * cells), "unit": "uncertainty maps/s", "cores": 1,
+                             "kind": "port", "seconds_per_map": t_cpu * cells,
+                             "sample": "%d cells of the reference's per-cell loop (oracle "
+                                       "restatement, numpy argmin per draw), extrapolated to %d "
+                                       "cells" % (len(idx), cells)}}
 
 
 def inverse_bench(eng, count):
@@ -231,7 +269,8 @@ def main():
         r64 = eng.simulate(cfg, n64, seed=78, fp64=True)
         out["fp64_kernel_packets_per_s"] = n64 / (r64.kernel_ms / 1e3)
     if not a.no_extra:
-        out["lookup_sweep"] = lookup_sweep(eng)
+        out["lookup_sweep"], sweep_res = lookup_sweep(eng)
+        out["lookup_tables"] = lookup_tables(sweep_res)
     if a.inverse:
         out["inverse"] = inverse_bench(eng, a.inverse)
     if a.copy_probe:

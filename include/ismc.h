@@ -190,6 +190,22 @@ int ismc_mlp_train(ismc_mlp* mlp, const int32_t* perm, int32_t epochs, int32_t s
                    float* epoch_loss, float* total_ms);
 int ismc_mlp_last_ms(ismc_mlp* mlp, float* ms);
 
+/* ---------------------------------------------------------------------
+ * Lookup-table uncertainty map.  Replaces the per-cell Monte Carlo loop
+ * of makelookup (Lookup/lookup.py:347-376, with find_nearest,
+ * lookup.py:296-299): for every cell (i, j) of P x P tables indexed
+ * [i (R)][j (T)] whose Rstd and Tstd are finite and >= 0, draw M samples
+ * R ~ N(Rvals[i], Rstd), T ~ N(Tvals[j], Tstd), look each up at the
+ * nearest grid point, and return 100 * std / centre of the mu_a and mu_s'
+ * values found (NaN elsewhere).  Rvals and Tvals must be the same uniform
+ * increasing grid.  All pointers are host memory; the call blocks.
+ * Normals from Philox4x32-10 with key = seed (Box-Muller, FP64).
+ * --------------------------------------------------------------------- */
+int ismc_table_uncertainty(int device, const double* mua, const double* mus, const double* rstd,
+                           const double* tstd, const double* Rvals, const double* Tvals, int32_t P,
+                           int32_t M, uint64_t seed, double* mua_unc, double* mus_unc);
+const char* ismc_table_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

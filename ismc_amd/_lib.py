@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libismc.so"
 
 EXPORTS = ("ismc_create", "ismc_destroy", "ismc_last_error", "ismc_device_info_get",
            "ismc_simulate", "ismc_simulate_batch", "ismc_simulate_packets", "ismc_last_launch",
-           "ismc_eval_physics", "ismc_validate_config", "ismc_copy_probe", "ismc_graph_stats")
+           "ismc_eval_physics", "ismc_validate_config", "ismc_copy_probe", "ismc_graph_stats",
+           "ismc_table_uncertainty", "ismc_table_last_error")
 
 
 class Sphere(ctypes.Structure):
@@ -106,6 +107,11 @@ def load():
         L.ismc_graph_stats.restype = ctypes.c_int
         L.ismc_copy_probe.argtypes = [vp, ctypes.c_int64, P(ctypes.c_float)]
         L.ismc_copy_probe.restype = ctypes.c_int
+        L.ismc_table_uncertainty.argtypes = [ctypes.c_int] + [vp] * 6 + [ctypes.c_int32, ctypes.c_int32,
+                                                                         ctypes.c_uint64, vp, vp]
+        L.ismc_table_uncertainty.restype = ctypes.c_int
+        L.ismc_table_last_error.argtypes = []
+        L.ismc_table_last_error.restype = ctypes.c_char_p
         _lib = L
         return L
 
