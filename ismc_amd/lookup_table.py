@@ -144,3 +144,37 @@ def save(path, tabs: Tables, mua_unc, mus_unc, extra: Optional[dict] = None):
          "mua_points": tabs.mua_points, "mus_points": tabs.mus_points}
     d.update(extra or {})
     np.savez(path, **d)
+
+
+def run(ini, seed=1234, engine=None, rng_seed=None, out_prefix=None):
+    """lookup.py's whole run (lookup.py:491-560) from an .ini: the simulated
+    grid over every input-uncertainty sample (one batched launch), the P x P
+    tables (P = [lookup] dimensions), their uncertainty maps (M =
+    simulation_distributions_samples draws per cell), and every
+    [experiment] measurement looked up with U =
+    measurement_distributions_samples draws.  Measurements are fractions in
+    the .ini and percent in the tables, as in the reference."""
+    from . import lookup as sweep_mod
+    L = ini.lookup
+    P = int(L.get("dimensions", 1000))
+    M = int(L.get("simulation_distributions_samples", 200))
+    U = int(L.get("measurement_distributions_samples", 1000))
+    sweep = sweep_mod.run_lookup(ini, seed=seed, engine=engine, rng_seed=rng_seed)
+    tabs = make_tables(sweep["R_d"], sweep["T_d"], sweep["R_d_std"], sweep["T_d_std"], sweep["mua"],
+                       sweep["mus_reduced"], P=P)
+    ua, us = table_uncertainty(tabs, M=M, seed=seed, device=engine.device if engine is not None else 0)
+    rng = np.random.default_rng(rng_seed)
+    results = []
+    if ini.M_R is not None and ini.M_T is not None:
+        n = max(len(ini.M_R.values), len(ini.M_T.values))
+        # a single value (or uncertainty) applies to every measurement; a
+        # missing uncertainty is 0
+        pick = lambda xs, k: xs[k] if len(xs) > 1 else (xs[0] if xs else 0.0)
+        for k in range(n):
+            results.append(measure(tabs, ua, us, 100 * pick(ini.M_R.values, k), 100 * pick(ini.M_R.uncs, k),
+                                   100 * pick(ini.M_T.values, k), 100 * pick(ini.M_T.uncs, k), U=U, rng=rng))
+    if out_prefix:
+        save(out_prefix + "_lookup_tables.npz", tabs, ua, us,
+             extra={"muas": sweep["mua"], "muss": sweep["mus_reduced"], "n": ini.n, "n_unc": ini.n_unc,
+                    "g": ini.g, "t": ini.t})
+    return {"tables": tabs, "mua_unc": ua, "mus_unc": us, "measurements": results, "sweep": sweep}

@@ -54,3 +54,26 @@ def test_full_size_table(engine):
     for c in cells:
         assert ua[c] == pytest.approx(ra[c], rel=1e-9, abs=1e-10)
         assert us[c] == pytest.approx(rs[c], rel=1e-9, abs=1e-10)
+
+
+def test_run_from_ini(engine, tmp_path):
+    import os
+    from ismc_amd import ini as ini_mod
+    here = os.path.dirname(os.path.abspath(__file__))
+    f = ini_mod.load(os.path.join(here, "golden", "sample_lookup.ini"))
+    out = LT.run(f, seed=3, engine=engine, rng_seed=1, out_prefix=str(tmp_path / "TEST"))
+    tabs = out["tables"]
+    assert tabs.mua.shape == (100, 100) and out["mua_unc"].shape == (100, 100)
+    assert len(out["measurements"]) == 2
+    assert all(len(m["lines"]) == 2 for m in out["measurements"])
+    d = np.load(str(tmp_path / "TEST_lookup_tables.npz"), allow_pickle=False)
+    for key in ("mua_lookup", "mus_lookup", "mua_lookup_uncertainty", "mus_lookup_uncertainty", "Rvals",
+                "Tvals", "mua_points", "muas"):
+        assert key in d
+    assert np.array_equal(d["mua_lookup"], tabs.mua.T, equal_nan=True)
+    # a simulated point read back through its own table cell
+    sw = out["sweep"]
+    r, t = 100 * sw["R_d"][1, 2], 100 * sw["T_d"][1, 2]
+    a, _, _ = LT.lookup(tabs.mua, tabs.Rvals, tabs.Tvals, r, t)
+    if np.isfinite(a):
+        assert abs(a / sw["mua"][1] - 1) < 0.15
