@@ -47,6 +47,197 @@ class _ParkMiller:
         return self.seed * 4.656612875e-10
 
 
+def _norm(v):
+    """Euclidean norm, squares summed in index order."""
+    s = 0.0
+    for a in v:
+        s = s + float(a) * float(a)
+    return math.sqrt(s)
+
+
+def _curve(l, d0, d1):
+    """Weights of q0, x, q1 for the point at arc length l on the quadratic
+    through q0 (at -d0), x (at 0) and q1 (at d1): Lagrange interpolation,
+    with the divisions in a fixed order."""
+    qa = l * (l - d1) / (d0 + d1) / d0
+    qb = -(l + d0) * (l - d1) / d1 / d0
+    qc = (l + d0) * l / d1 / (d0 + d1)
+    return qa, qb, qc
+
+
+def _hypot(x, y):
+    """sqrt(x^2 + y^2) as a sqrt(1 + (b/a)^2) with a = max(|x|, |y|)."""
+    a, b = (abs(y), abs(x)) if abs(x) < abs(y) else (abs(x), abs(y))
+    return 0.0 if a == 0.0 else a * math.sqrt(1.0 + (b / a) * (b / a))
+
+
+def _minfit(a, tol):
+    """Singular value decomposition by Golub and Reinsch's method in the form
+    Brent gives for PRAXIS (Algorithms for Minimization without Derivatives,
+    ch. 7, procedure minfit): Householder reduction of the n x n matrix `a`
+    (list of rows) to bidiagonal form, accumulation of the right-hand
+    transformations, and implicitly shifted QR on the bidiagonal.  On return
+    `a` holds the right singular vectors as columns; the singular values (not
+    sorted, all >= 0) are returned.  Rotations are applied in the same order
+    and with the same sign conventions as that procedure, so its output is
+    reproduced to the last bit."""
+    n = len(a)
+    if n == 1:
+        q = [a[0][0]]
+        a[0][0] = 1.0
+        return q
+    e = [0.0] * n
+    q = [0.0] * n
+    g = x = 0.0
+    # Householder reduction to bidiagonal form
+    for i in range(n):
+        e[i] = g
+        l = i + 1
+        s = 0.0
+        for r in range(i, n):
+            s = s + a[r][i] * a[r][i]
+        g = 0.0
+        if tol <= s:
+            f = a[i][i]
+            g = math.sqrt(s)
+            if 0.0 <= f:
+                g = -g
+            h = f * g - s
+            a[i][i] = f - g
+            for j in range(l, n):
+                f = 0.0
+                for r in range(i, n):
+                    f = f + a[r][i] * a[r][j]
+                f = f / h
+                for r in range(i, n):
+                    a[r][j] = a[r][j] + f * a[r][i]
+        q[i] = g
+        s = 0.0
+        for j in range(l, n):
+            s = s + a[i][j] * a[i][j]
+        g = 0.0
+        if tol <= s:
+            if i < n - 1:
+                f = a[i][i + 1]
+            g = math.sqrt(s)
+            if 0.0 <= f:
+                g = -g
+            h = f * g - s
+            if i < n - 1:
+                a[i][i + 1] = f - g
+                for c in range(l, n):
+                    e[c] = a[i][c] / h
+                for j in range(l, n):
+                    s = 0.0
+                    for c in range(l, n):
+                        s = s + a[j][c] * a[i][c]
+                    for c in range(l, n):
+                        a[j][c] = a[j][c] + s * e[c]
+        y = abs(q[i]) + abs(e[i])
+        x = x if y < x else y
+    # accumulate the right-hand transformations
+    a[n - 1][n - 1] = 1.0
+    g = e[n - 1]
+    l = n - 1
+    for i in range(n - 2, -1, -1):
+        if g != 0.0:
+            h = a[i][i + 1] * g
+            for r in range(l, n):
+                a[r][i] = a[i][r] / h
+            for j in range(l, n):
+                s = 0.0
+                for c in range(l, n):
+                    s = s + a[i][c] * a[c][j]
+                for r in range(l, n):
+                    a[r][j] = a[r][j] + s * a[r][i]
+        for c in range(l, n):
+            a[i][c] = 0.0
+        for r in range(l, n):
+            a[r][i] = 0.0
+        a[i][i] = 1.0
+        g = e[i]
+        l = i
+    # diagonalise the bidiagonal form
+    eps = MACHEP * x
+    for k in range(n - 1, -1, -1):
+        its = 0
+        while True:
+            its += 1
+            if its > 30:
+                raise RuntimeError("minfit: the QR iteration did not converge")
+            split = False
+            for l in range(k, -1, -1):
+                if abs(e[l]) <= eps:
+                    split = True
+                    break
+                if l > 0 and abs(q[l - 1]) <= eps:
+                    break
+            if not split:
+                # cancel e[l] (q[l - 1] is negligible)
+                c, s = 0.0, 1.0
+                for i in range(l, k + 1):
+                    f = s * e[i]
+                    e[i] = c * e[i]
+                    if abs(f) <= eps:
+                        break
+                    g = q[i]
+                    h = _hypot(f, g)
+                    q[i] = h
+                    if h == 0.0:
+                        g = h = 1.0
+                    c = g / h
+                    s = -f / h
+            z = q[k]
+            if l == k:
+                if z < 0.0:
+                    q[k] = -z
+                    for r in range(n):
+                        a[r][k] = -a[r][k]
+                break
+            # shift from the bottom 2 x 2 minor
+            x = q[l]
+            y = q[k - 1]
+            g = e[k - 1]
+            h = e[k]
+            f = ((y - z) * (y + z) + (g - h) * (g + h)) / (2.0 * h * y)
+            g = _hypot(f, 1.0)
+            f = ((x - z) * (x + z) + h * (y / (f - g if f < 0.0 else f + g) - h)) / x
+            # QR transformation
+            c = s = 1.0
+            for i in range(l + 1, k + 1):
+                g = e[i]
+                y = q[i]
+                h = s * g
+                g = g * c
+                z = _hypot(f, h)
+                e[i - 1] = z
+                if z == 0.0:
+                    f = z = 1.0
+                c = f / z
+                s = h / z
+                f = x * c + g * s
+                g = -x * s + g * c
+                h = y * s
+                y = y * c
+                for r in range(n):
+                    xr = a[r][i - 1]
+                    zr = a[r][i]
+                    a[r][i - 1] = xr * c + zr * s
+                    a[r][i] = -xr * s + zr * c
+                z = _hypot(f, h)
+                q[i - 1] = z
+                if z == 0.0:
+                    f = z = 1.0
+                c = f / z
+                s = h / z
+                f = c * g + s * y
+                x = -s * g + c * y
+            e[l] = 0.0
+            e[k] = f
+            q[k] = x
+    return q
+
+
 def praxis(x0, t0, h0, max_evals=100000):
     """Generator form of PRAXIS; yields points, receives f at each point,
     returns (x, f(x), evaluations) when converged."""
@@ -92,9 +283,7 @@ def praxis(x0, t0, h0, max_evals=100000):
         if j >= 0:
             pt = x + lam * V[:, j]
         else:
-            qa = (lam * (lam - Q["qd1"])) / (Q["qd0"] * (Q["qd0"] + Q["qd1"]))
-            qb = ((lam + Q["qd0"]) * (Q["qd1"] - lam)) / (Q["qd0"] * Q["qd1"])
-            qc = (lam * (lam + Q["qd0"])) / (Q["qd1"] * (Q["qd0"] + Q["qd1"]))
+            qa, qb, qc = _curve(lam, Q["qd0"], Q["qd1"])
             Q["qa"], Q["qb"], Q["qc"] = qa, qb, qc
             pt = (qa * q0 + qb * x) + qc * q1
         return (yield from evaluate(pt))
@@ -107,7 +296,7 @@ def praxis(x0, t0, h0, max_evals=100000):
         xm = 0.0
         fm = f0 = fx
         dz = d2 < MACHEP
-        s = math.sqrt(float(np.dot(x, x)))
+        s = _norm(x)
         temp = dmin if dz else d2
         t2_ = m4 * math.sqrt(abs(fx) / temp + s * ldt) + m2 * ldt
         s = m4 * s + t
@@ -180,13 +369,11 @@ def praxis(x0, t0, h0, max_evals=100000):
         old_x = x.copy()
         x = q1.copy()
         q1 = old_x
-        Q["qd1"] = math.sqrt(float(np.sum((q1 - x) ** 2)))
+        Q["qd1"] = _norm(x - q1)
         lam = Q["qd1"]
         if Q["qd0"] > 0.0 and Q["qd1"] > 0.0 and st["nl"] >= 3 * n * n:
             _, lam = yield from line_min(-1, 2, 0.0, lam, Q["qf1"], True)
-            qa = (lam * (lam - Q["qd1"])) / (Q["qd0"] * (Q["qd0"] + Q["qd1"]))
-            qb = ((lam + Q["qd0"]) * (Q["qd1"] - lam)) / (Q["qd0"] * Q["qd1"])
-            qc = (lam * (lam + Q["qd0"])) / (Q["qd1"] * (Q["qd0"] + Q["qd1"]))
+            qa, qb, qc = _curve(lam, Q["qd0"], Q["qd1"])
         else:
             fx = Q["qf1"]
             qa, qb, qc = 0.0, 0.0, 1.0
@@ -237,7 +424,7 @@ def praxis(x0, t0, h0, max_evals=100000):
             step = x - y
             x = y.copy()
             y = step
-            lds = math.sqrt(float(np.dot(step, step)))
+            lds = _norm(step)
             if lds > small:
                 for i in range(kl - 1, k - 1, -1):
                     V[:, i + 1] = V[:, i]
@@ -251,7 +438,7 @@ def praxis(x0, t0, h0, max_evals=100000):
             ldt = ldfac * ldt
             if ldt < lds:
                 ldt = lds
-            t2 = m2 * math.sqrt(float(np.dot(x, x))) + t
+            t2 = m2 * _norm(x) + t
             if ldt > 0.5 * t2:
                 kt = -1
             kt += 1
@@ -263,10 +450,11 @@ def praxis(x0, t0, h0, max_evals=100000):
             d[i] = 1.0 / math.sqrt(d[i])
             dn = max(dn, d[i])
         V = V * (d / dn)[None, :]
-        # new orthogonal directions: singular vectors of the scaled direction matrix
-        _, sv, wt = np.linalg.svd(V.T)
-        V = wt.T.copy()
-        d = sv.copy()
+        # new orthogonal directions: singular vectors of the scaled direction
+        # matrix (minfit on its transpose returns them as columns)
+        A = [[float(V[j, i]) for j in range(n)] for i in range(n)]
+        d = np.array(_minfit(A, vsmall))
+        V = np.array(A)
         for i in range(n):
             dni = dn * d[i]
             if dni > large:
@@ -274,10 +462,17 @@ def praxis(x0, t0, h0, max_evals=100000):
             elif dni < small:
                 d[i] = vlarge
             else:
-                d[i] = 1.0 / (dni * dni)
-        order = np.argsort(-d, kind="stable")
-        d = d[order]
-        V = V[:, order]
+                d[i] = 1.0 / dni / dni
+        # order by decreasing value: selection sort, swapping the first
+        # largest remaining entry into place
+        for j1 in range(n - 1):
+            j3 = j1
+            for j2 in range(j1 + 1, n):
+                if d[j3] < d[j2]:
+                    j3 = j2
+            if j1 != j3:
+                d[[j1, j3]] = d[[j3, j1]]
+                V[:, [j1, j3]] = V[:, [j3, j1]]
         dmin = max(d[n - 1], small)
         illc = m2 * d[0] > dmin
 
