@@ -1,0 +1,63 @@
+"""The surrogate-seeded inverse solver on the GPU with the packaged network:
+what the trained network gains over the same solver without it, and the
+MC judge with its fallback."""
+import numpy as np
+import pytest
+
+from ismc_amd import inverse as I
+from ismc_amd.surrogate import Surrogate, packaged_weights
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup(engine):
+    sur = Surrogate.load(packaged_weights())
+    meas = I.make_measurements(256, 1_000_000, seed=7, engine=engine)
+    probs = [I.problem(r, t) for r, t in zip(meas["R_d"], meas["T_d"])]
+    return sur, meas, probs
+
+
+def test_network_gain(engine, setup):
+    sur, meas, probs = setup
+    truth = np.stack([meas["mu_a"], meas["mus_reduced"]], 1)
+    # with the network: its seeds, then 6 batched MC launches of defect
+    # correction with its Jacobian
+    seeds = I.surrogate_seeds(sur, meas["R_d"], meas["T_d"])
+    sol_s, nev_s, _ = I.refine_newton(probs, seeds, sur, packets=100_000, engine=engine, stream_base=1000)
+    # without it: the same batched MC minimization (PRAXIS on the MC cost,
+    # 1e5 packets per evaluation) from the generic guess, up to 400 evaluations
+    sol_g, nev_g, _ = I.refine_batch(probs, [I.GENERIC_GUESS] * len(probs), packets=100_000, step=1.0,
+                                     max_evals=400, engine=engine, stream_base=1000)
+    ea_s, es_s = I.errors(sol_s, truth[:, 0], truth[:, 1])
+    ea_g, es_g = I.errors(sol_g, truth[:, 0], truth[:, 1])
+    cost_truth = I.fit_cost(probs, truth, 1_000_000, engine)
+    fit_s = np.mean(I.fit_cost(probs, sol_s, 1_000_000, engine) <= cost_truth + I.FIT_TOL)
+    fit_g = np.mean(I.fit_cost(probs, sol_g, 1_000_000, engine) <= cost_truth + I.FIT_TOL)
+    print("with the network: %.0f MC evaluations per solve, median mu_a err %.3f, mu_s' err %.3f, fit %.3f"
+          % (nev_s.mean(), np.median(ea_s), np.median(es_s), fit_s))
+    print("without: %.0f MC evaluations per solve, median mu_a err %.3f, mu_s' err %.3f, fit %.3f"
+          % (nev_g.mean(), np.median(ea_g), np.median(es_g), fit_g))
+    assert nev_s.mean() * 10 < nev_g.mean()
+    assert np.median(es_s) < np.median(es_g) and np.median(ea_s) < np.median(ea_g)
+    assert fit_s > fit_g and fit_s > 0.7
+
+
+def test_judge_flags_misses_and_fallback_repairs_them(engine, setup):
+    sur, meas, probs = setup
+    k = 32
+    sub = probs[:k]
+    seeds = I.surrogate_seeds(sur, meas["R_d"][:k], meas["T_d"][:k])
+    sol, _, _ = I.refine_newton(sub, seeds, sur, packets=100_000, engine=engine, stream_base=2000)
+    # corrupt every other solution: absorption x5, scattering / 3
+    bad = sol.copy()
+    bad[::2] *= [5.0, 1 / 3.0]
+    miss, rR, rT, _ = I.check(sub, bad, engine=engine)
+    assert miss[::2].all()
+    # the fallback (batched MC PRAXIS from the judged point) brings them back
+    fb, _, _ = I.refine_batch([sub[i] for i in range(0, k, 2)], bad[::2], packets=100_000, step=0.3,
+                              tol=3e-3, max_evals=150, engine=engine, stream_base=3000)
+    before = I.fit_cost([sub[i] for i in range(0, k, 2)], bad[::2], 1_000_000, engine)
+    after = I.fit_cost([sub[i] for i in range(0, k, 2)], fb, 1_000_000, engine)
+    print("fit cost of the corrupted solutions: median %.3f -> %.4f" % (np.median(before), np.median(after)))
+    assert np.median(after) < 0.1 * np.median(before)
