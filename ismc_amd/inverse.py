@@ -293,14 +293,15 @@ def fit_cost(problems, sol, packets, eng, seed=13):
 
 def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_packets=1_000_000,
         plain_count=8, plain_packets=1_000_000, seed=0, engine: Optional[Engine] = None,
-        methods=("newton", "praxis"), with_ablation=True, log=print):
+        methods=("newton", "praxis"), with_ablation=True, log=print, noise=0.01, plain_tol=3e-3):
     """Surrogate-seeded batched solves vs plain MC minimization on the same
-    GPU, over `count` synthetic measurements (1 % noise on R_d and T_d)."""
+    GPU, over `count` synthetic measurements (`noise` relative Gaussian
+    noise on R_d and T_d)."""
     eng = engine or default_engine()
-    meas = make_measurements(count, meas_packets, seed=seed, engine=eng)
+    meas = make_measurements(count, meas_packets, noise=noise, seed=seed, engine=eng)
     probs = [problem(r, t) for r, t in zip(meas["R_d"], meas["T_d"])]
-    out = {"count": count, "meas_packets": meas_packets, "refine_packets": refine_packets,
-           "check_packets": check_packets}
+    out = {"count": count, "noise": noise, "meas_packets": meas_packets, "refine_packets": refine_packets,
+           "check_packets": check_packets, "plain_packets": plain_packets, "plain_tol": plain_tol}
     log("measurements ready (%d at %d packets)" % (count, meas_packets))
     # the truth's own misfit to the noisy measurement (MC at check_packets)
     cost_truth = fit_cost(probs, np.stack([meas["mu_a"], meas["mus_reduced"]], 1), check_packets, eng)
@@ -336,7 +337,8 @@ def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_p
         log("  unseeded: %.1f solves/s, %.1f evals/solve" % (count / t2, nev2.mean()))
     pe, ptimes, pevals = [], [], []
     for k in range(plain_count):
-        x, nf, secs, _ = solve_plain(probs[k], packets=plain_packets, engine=eng, stream=120_000_000 + k)
+        x, nf, secs, _ = solve_plain(probs[k], packets=plain_packets, tol=plain_tol, engine=eng,
+                                     stream=120_000_000 + k)
         pe.append(x)
         ptimes.append(secs)
         pevals.append(nf)

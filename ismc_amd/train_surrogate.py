@@ -31,6 +31,11 @@ def main(argv=None):
     ap.add_argument("--json", default="")
     ap.add_argument("--no-ablation", action="store_true", help="skip the unseeded batched solves")
     ap.add_argument("--methods", default="newton,praxis")
+    ap.add_argument("--noise", type=float, default=0.01, help="relative noise on the synthetic measurements")
+    ap.add_argument("--meas-packets", type=int, default=1_000_000)
+    ap.add_argument("--check-packets", type=int, default=1_000_000)
+    ap.add_argument("--plain-packets", type=int, default=1_000_000)
+    ap.add_argument("--plain-tol", type=float, default=3e-3)
     a = ap.parse_args(argv)
     log = lambda *x: print(*x, file=sys.stderr, flush=True)
     eng = Engine(0)
@@ -61,7 +66,9 @@ def main(argv=None):
         result["inverse"] = inverse.run(sur, count=a.inverse, plain_count=a.plain,
                                         refine_packets=a.refine_packets, engine=eng, log=log,
                                         methods=tuple(a.methods.split(",")),
-                                        with_ablation=not a.no_ablation)
+                                        with_ablation=not a.no_ablation, noise=a.noise,
+                                        meas_packets=a.meas_packets, check_packets=a.check_packets,
+                                        plain_packets=a.plain_packets, plain_tol=a.plain_tol)
     out = json.dumps(result)
     print(out, flush=True)
     if a.json:
