@@ -154,15 +154,22 @@ def test_large_run_properties(engine):
     assert ev["entrances"] == ev["sphere_decisions"] and ev["reinjections"] < ev["entrances"]
 
 
-def test_fp32_vs_fp64_kernel_deviation(engine):
-    cfg = slab_between_slides()
-    n = 4_000_000
+@pytest.mark.parametrize("name,cfg", [("slab_between_slides", slab_between_slides()),
+                                      ("three_layer", three_layer_sample())])
+def test_fp32_vs_fp64_kernel_deviation(engine, name, cfg):
+    # same Philox streams: the difference is the transport precision alone
+    n = 1 << 26
     f32 = engine.simulate(cfg, n, seed=55)
     f64 = engine.simulate(cfg, n, seed=55, fp64=True)
     for k in f32.mean:
         dev = f32.mean[k] - f64.mean[k]
-        print("FP32-FP64 %-3s %+.3e (SE %.3e)" % (k, dev, f64.se[k]))
+        print("%s FP32-FP64 %-3s %+.3e (SE %.3e, %+.4f SE)" % (name, k, dev, f64.se[k], dev / f64.se[k]))
         assert abs(dev) < 3 * f64.se[k] + 1e-9
+    _, a = engine.simulate_packets(cfg, 1_000_000, seed=7)
+    _, b = engine.simulate_packets(cfg, 1_000_000, seed=7, fp64=True)
+    same = np.mean(np.argmax(a[:, 1:5] > 0, 1) == np.argmax(b[:, 1:5] > 0, 1))
+    print("%s packets ending in the same tally: %.6f" % (name, same))
+    assert same > 0.999
 
 
 def test_cuda_roulette_compat_only_loses_weight(engine):
