@@ -269,6 +269,8 @@ struct ismc_mlp {
   float* slabs = nullptr;               // split-K partials
   float* Tb = nullptr;                  // batch targets
   float* lossp = nullptr;
+  float* lossd = nullptr;               // per-step loss partials of a training call
+  size_t lossd_cap = 0;
   int* idx = nullptr;
   float* stage_x = nullptr;             // [maxB][n_in] upload staging
   float* stage_y = nullptr;             // [maxB][n_out]
@@ -317,7 +319,7 @@ static int gemm(ismc_mlp* p, int M, int N, int K, const bf16_t* A, int lda, cons
 
 static void mlp_free(ismc_mlp* p) {
   void* bufs[] = {p->w, p->g, p->m, p->v, p->mask, p->wb, p->X0, p->Y, p->dA, p->dB,
-                  p->slabs, p->Tb, p->lossp, p->idx, p->ds_x, p->ds_y, p->stage_x, p->stage_y};
+                  p->slabs, p->Tb, p->lossp, p->lossd, p->idx, p->ds_x, p->ds_y, p->stage_x, p->stage_y};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto z : p->Z)
@@ -634,8 +636,14 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
   MCHECK(p, hipSetDevice(p->device));
   const int nb = (B + 255) / 256;
   std::vector<float> lp((size_t)nb * steps_per_epoch);
-  float* lossd = nullptr;
-  if ((rc = dalloc(p, &lossd, (size_t)nb * steps_per_epoch))) return rc;
+  if (p->lossd_cap < lp.size()) {
+    if (p->lossd) (void)hipFree(p->lossd);
+    p->lossd = nullptr;
+    p->lossd_cap = 0;
+    if ((rc = dalloc(p, &p->lossd, lp.size()))) return rc;
+    p->lossd_cap = lp.size();
+  }
+  float* lossd = p->lossd;
   float ms_total = 0.f;
   for (int e = 0; e < epochs; ++e) {
     MCHECK(p, hipMemcpyAsync(p->idx, perm + (size_t)e * steps_per_epoch * B,
@@ -646,10 +654,7 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
       gather_batch<<<nb, 256, 0, p->stream>>>(p->ds_x, p->n_in, p->ds_y, p->n_out, p->idx + (size_t)s * B, B,
                                               p->in_pad, p->X0, p->Tb);
       MCHECK(p, hipGetLastError());
-      if ((rc = forward(p, B)) || (rc = backward(p, B, nullptr, lossd + (size_t)s * nb))) {
-        (void)hipFree(lossd);
-        return rc;
-      }
+      if ((rc = forward(p, B)) || (rc = backward(p, B, nullptr, lossd + (size_t)s * nb))) return rc;
       p->steps++;
       const float bc1 = 1.f - powf(b1, (float)p->steps), bc2 = 1.f - powf(b2, (float)p->steps);
       const float lr_t = lr * sqrtf(bc2) / bc1;
@@ -673,7 +678,6 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
     ms_total += ms;
     if (epoch_loss) epoch_loss[e] = (float)(esum / steps_per_epoch);
   }
-  (void)hipFree(lossd);
   if (total_ms) *total_ms = ms_total;
   return 0;
 }

@@ -1129,17 +1129,24 @@ int ismc_copy_probe(ismc_ctx* c, int64_t bytes, float* kernel_ms) {
   }
   const size_t n = (size_t)bytes / 16;
   const int grid = c->prop.multiProcessorCount * 8;
-  copy_probe_kernel<<<grid, 256, 0, c->stream>>>((const uint4*)b, (uint4*)a, n);  // warm
-  HIPCHECK(c, hipEventRecord(c->e0, c->stream));
-  copy_probe_kernel<<<grid, 256, 0, c->stream>>>((const uint4*)a, (uint4*)b, n);
-  HIPCHECK(c, hipEventRecord(c->e1, c->stream));
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  float ms = 0.f;
-  HIPCHECK(c, hipEventElapsedTime(&ms, c->e0, c->e1));
-  if (kernel_ms) *kernel_ms = ms;
+  // one exit path, so that both buffers are freed whatever fails
+  auto timed_copy = [&]() -> int {
+    copy_probe_kernel<<<grid, 256, 0, c->stream>>>((const uint4*)b, (uint4*)a, n);  // warm
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipEventRecord(c->e0, c->stream));
+    copy_probe_kernel<<<grid, 256, 0, c->stream>>>((const uint4*)a, (uint4*)b, n);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipEventRecord(c->e1, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHECK(c, hipEventElapsedTime(&ms, c->e0, c->e1));
+    if (kernel_ms) *kernel_ms = ms;
+    return 0;
+  };
+  const int rc = timed_copy();
   (void)hipFree(a);
   (void)hipFree(b);
-  return 0;
+  return rc;
 }
 
 }  // extern "C"
