@@ -367,7 +367,8 @@ static int backward(ismc_mlp* p, int B, float* loss_host, float* loss_dev = null
     const size_t nw = (size_t)Ly.K * Ly.N;
     reduce_slabs<<<(unsigned)((nw + 255) / 256), 256, 0, p->stream>>>(p->slabs, ks, nw, p->g + Ly.w_off);
     // bias gradient: column sums over kBiasSlabs row slabs (the split-K slabs
-    // above are consumed by now and hold the partials), then a fixed-order sum
+    // above are consumed by now and hold the partials: kBiasSlabs x N floats
+    // fit in k_split x K x N), then a fixed-order sum
     {
       const int rows = (B + kBiasSlabs - 1) / kBiasSlabs;
       const int nslab = (B + rows - 1) / rows;

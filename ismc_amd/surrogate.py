@@ -64,10 +64,11 @@ def sample_config(mu_a, mu_s, g, n, t):
 
 
 def generate_dataset(n_samples, packets, seed=0, engine: Optional[Engine] = None,
-                     points_per_launch=16384, progress=None):
+                     points_per_launch=16384, progress=None, autotune=True):
     """MC labels for n_samples inputs: one batched launch per
-    points_per_launch samples (one work-queue chunk per sample).  Returns
-    (inputs dict, tallies [n, 4] in OUTPUTS order, seconds, packets)."""
+    points_per_launch samples (one work-queue chunk per sample unless
+    `autotune` picks the launch shape; labels are identical either way).
+    Returns (inputs dict, tallies [n, 4] in OUTPUTS order, seconds, packets)."""
     eng = engine or default_engine()
     inp = sample_inputs(n_samples, seed)
     tal = np.zeros((n_samples, 4))
@@ -77,7 +78,8 @@ def generate_dataset(n_samples, packets, seed=0, engine: Optional[Engine] = None
         cfgs = [sample_config(inp["mu_a"][i], inp["mu_s"][i], inp["g"][i], inp["n"][i], inp["t"][i])
                 for i in range(s, e)]
         res = eng.simulate_batch(cfgs, packets, seed=seed + 7, streams=np.arange(s, e, dtype=np.uint32),
-                                 chunk_packets=min(int(packets), 1 << 20))
+                                 chunk_packets=0 if autotune else min(int(packets), 1 << 20),
+                                 autotune=autotune)
         for k, r in enumerate(res):
             m = r.mean
             tal[s + k] = (m["R_d"], m["T_d"], m["T_u"], m["R_d"] + m["R_s"])
