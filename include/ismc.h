@@ -146,8 +146,9 @@ int ismc_last_launch(ismc_ctx* ctx, float* kernel_ms, int32_t* grid_blocks,
                      int32_t* block_threads, int64_t* n_chunks);
 
 /* How many launch graphs were captured and how many graph replays ran
- * (ismc_tune.use_graph = 1 replays a captured graph when a batch call
- * repeats the shape, buffers and seed of an earlier one). */
+ * (with ismc_tune.use_graph = 1 a batch call's launch sequence is captured
+ * the second time its shape, buffers and seed come, and replayed from the
+ * third on; the first runs without a graph). */
 int ismc_graph_stats(ismc_ctx* ctx, int64_t* captures, int64_t* launches);
 
 /* Evaluate the device physics helpers on host inputs (parity aid):

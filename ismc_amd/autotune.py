@@ -26,7 +26,9 @@ from typing import Dict, Optional, Sequence, Tuple
 
 import numpy as np
 
-CHUNKS = (512, 1024, 2048, 4096, 8192, 16384, 65536)
+# 0 = the library's default: 4096 packets, fewer for a launch too small to
+# give every resident wave four chunks
+CHUNKS = (0, 512, 1024, 2048, 4096, 8192, 16384, 65536)
 IDLE = (4, 8, 16, 32)
 BOUND = (4, 8, 16, 32)
 BLOCKS = (0, 2, 4)
@@ -83,7 +85,7 @@ class AutoTuner:
         total = float(counts.sum())
         streams = np.arange(npts, dtype=np.uint32)
         max_chunk = int(counts.max()) if npts else 1
-        chunks = [c for c in self.chunks if c <= max(max_chunk, self.chunks[0])]
+        chunks = [c for c in self.chunks if c <= max(max_chunk, 512)]
         table = []
         # first launch of a shape pays module load and buffer growth: discard one
         self.engine.simulate_batch(cfgs, counts, seed=seed, streams=streams, fp64=fp64)

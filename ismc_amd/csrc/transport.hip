@@ -644,6 +644,7 @@ struct ismc_ctx {
     hipGraphExec_t exec = nullptr;
   };
   std::vector<Graph> graphs;
+  std::vector<std::vector<uint64_t>> seen;  // shapes run once, captured when they come back
   int64_t graph_launches = 0;
   int64_t graph_captures = 0;
 };
@@ -654,6 +655,7 @@ static void drop_graphs(ismc_ctx* c) {
     if (g.graph) (void)hipGraphDestroy(g.graph);
   }
   c->graphs.clear();
+  c->seen.clear();
 }
 
 static int set_err(ismc_ctx* c, int code, const char* fmt, const char* a = "", long long b = 0) {
@@ -813,8 +815,29 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
                       const uint64_t* first, const uint32_t* streams, int32_t n_points,
                       uint64_t seed, uint32_t flags, const ismc_tune* tune, ismc_tally* out,
                       double* per_packet_host) {
-  const int chunk = (tune && tune->chunk_packets > 0) ? tune->chunk_packets : c->chunk_packets;
   const int bpc = (tune && tune->blocks_per_cu > 0) ? tune->blocks_per_cu : c->blocks_per_cu;
+  // persistent grid: as many blocks as stay resident (occupancy), unless the
+  // caller asks for a given number per CU
+  int resident = 0;
+  if (per_packet_host)
+    HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, true>, kBlock, 0));
+  else
+    HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, false>, kBlock, 0));
+  const int per_cu = (tune && tune->blocks_per_cu > 0) ? bpc : std::max(1, resident);
+  // packets per queue chunk: the caller's, else the default, lowered for a
+  // small launch to give every resident wave at least four chunks (a wave
+  // runs a chunk's packets itself, so a few large chunks of long histories
+  // would leave most of the chip idle while they finish)
+  int chunk = c->chunk_packets;
+  if (tune && tune->chunk_packets > 0) {
+    chunk = tune->chunk_packets;
+  } else {
+    uint64_t requested = 0;
+    for (int p = 0; p < n_points; ++p) requested += std::min<uint64_t>(npk[p], 1ull << 32);
+    const uint64_t waves = (uint64_t)c->prop.multiProcessorCount * per_cu * kWavesPerBlock;
+    chunk = (int)std::min<uint64_t>((uint64_t)c->chunk_packets,
+                                    std::max<uint64_t>(64, requested / (4 * waves)));
+  }
   // chunk list
   std::vector<Chunk> chunks;
   uint64_t total = 0;
@@ -868,14 +891,6 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   a.bound_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->bound_lanes > 0) ? tune->bound_lanes
                                                                                    : c->bound_lanes));
 
-  // persistent grid: as many blocks as stay resident (occupancy), unless the
-  // caller asks for a given number per CU
-  int resident = 0;
-  if (per_packet_host)
-    HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, true>, kBlock, 0));
-  else
-    HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, false>, kBlock, 0));
-  const int per_cu = (tune && tune->blocks_per_cu > 0) ? bpc : std::max(1, resident);
   const uint64_t waves_useful = std::max<uint64_t>(1, (total + 63) / 64);
   const uint64_t blocks_useful = (waves_useful + kWavesPerBlock - 1) / kWavesPerBlock;
   const int grid = (int)std::max<uint64_t>(
@@ -904,17 +919,34 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
     return 0;
   };
   const bool use_graph = tune && tune->use_graph && !per_packet_host;
+  ismc_ctx::Graph* hit = nullptr;
   if (use_graph) {
     const std::vector<uint64_t> key = {
         sizeof(R), (uint64_t)n_points, (uint64_t)chunks.size(), (uint64_t)grid,
         (uint64_t)(uintptr_t)c->points.p, (uint64_t)(uintptr_t)c->chunks.p,
         (uint64_t)(uintptr_t)c->slots.p, (uint64_t)(uintptr_t)c->counter.p,
         (uint64_t)(uintptr_t)c->h_stage, pbytes, cbytes, sbytes, seed};
-    ismc_ctx::Graph* hit = nullptr;
     for (auto& g : c->graphs)
       if (g.key == key) hit = &g;
+    // a shape is captured the second time it comes: one-off shapes (a
+    // solver round with a new number of pending points) run eagerly
+    bool capture = false;
     if (!hit) {
-      if (c->graphs.size() >= 8) drop_graphs(c);
+      auto it = std::find(c->seen.begin(), c->seen.end(), key);
+      capture = it != c->seen.end();
+      if (capture) {
+        c->seen.erase(it);
+      } else {
+        if (c->seen.size() >= 32) c->seen.erase(c->seen.begin());
+        c->seen.push_back(key);
+      }
+    }
+    if (capture) {
+      if (c->graphs.size() >= 8) {  // evict the oldest
+        if (c->graphs.front().exec) (void)hipGraphExecDestroy(c->graphs.front().exec);
+        if (c->graphs.front().graph) (void)hipGraphDestroy(c->graphs.front().graph);
+        c->graphs.erase(c->graphs.begin());
+      }
       ismc_ctx::Graph g;
       g.key = key;
       HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -933,6 +965,8 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
       hit = &c->graphs.back();
       c->graph_captures++;
     }
+  }
+  if (use_graph && hit) {
     // the graph's own duration (uploads + clear + transport + download)
     HIPCHECK(c, hipEventRecord(c->e0, c->stream));
     HIPCHECK(c, hipGraphLaunch(hit->exec, c->stream));

@@ -12,6 +12,9 @@ from . import _lib
 from .config import SampleConfig
 
 TALLIES = ("A", "R_d", "R_s", "T_d", "T_u")
+# numpy views of the C structs (include/ismc.h)
+CONFIG_DTYPE = np.dtype(_lib.Config)
+TALLY_DTYPE = np.dtype(_lib.Tally)
 EVENTS = ("interactions", "boundary_hits", "entrances", "sphere_decisions", "reinjections")
 
 
@@ -131,10 +134,11 @@ class Engine:
                        chunk_packets: int = 0, use_graph: bool = False, autotune: bool = False,
                        idle_lanes: int = 0, bound_lanes: int = 0):
         """Many configurations, uneven packet counts, one persistent launch.
-        use_graph replays a captured hipGraph of the launch sequence when the
-        call repeats an earlier one's shape and seed (the uploads inside the
-        graph read the new configurations from pinned staging).  autotune
-        takes blocks_per_cu and chunk_packets from this engine's AutoTuner
+        use_graph captures the launch sequence as a hipGraph the second time
+        a call repeats an earlier one's shape and seed, and replays it from
+        then on (the uploads inside the graph read the new configurations
+        from pinned staging); a shape seen once runs without a graph.
+        autotune takes the launch shape from this engine's AutoTuner
         (timed on first use per workload class; tallies are identical for
         every launch shape)."""
         npts = len(cfgs)
@@ -166,6 +170,32 @@ class Engine:
         _lib.check(self._ctx, rc, "ismc_simulate_batch")
         ms = self.last_launch()["kernel_ms"]
         return [_result(outs[i], ms) for i in range(npts)]
+
+    def simulate_batch_array(self, configs: np.ndarray, n_packets, seed: int = 1234, streams=None,
+                             fp64: bool = False, use_graph: bool = False):
+        """simulate_batch on configurations already laid out as a numpy array
+        of the C struct (dtype CONFIG_DTYPE), for callers that patch a few
+        fields of many copies of a template.  Returns (sums [P, 5],
+        sums of squares [P, 5], packets [P]) as arrays, in TALLIES order."""
+        npts = int(configs.shape[0])
+        if npts == 0:
+            z = np.zeros((0, 5))
+            return z, z, np.zeros(0, dtype=np.uint64)
+        if configs.dtype != CONFIG_DTYPE or not configs.flags.c_contiguous:
+            raise ValueError("configs must be a C-contiguous array of CONFIG_DTYPE")
+        npk = np.ascontiguousarray(np.broadcast_to(np.asarray(n_packets, dtype=np.uint64), (npts,)))
+        st = None
+        if streams is not None:
+            st = np.ascontiguousarray(np.broadcast_to(np.asarray(streams, dtype=np.uint32), (npts,)))
+        tune = _lib.Tune(0, 0, int(bool(use_graph)), 0, 0, 0)
+        outs = np.zeros(npts, dtype=TALLY_DTYPE)
+        P64 = ctypes.POINTER(ctypes.c_uint64)
+        rc = self._L.ismc_simulate_batch(
+            self._ctx, configs.ctypes.data_as(ctypes.POINTER(_lib.Config)), npk.ctypes.data_as(P64), None,
+            st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if st is not None else None, npts, int(seed),
+            self._flags(fp64, False), ctypes.byref(tune), outs.ctypes.data_as(ctypes.POINTER(_lib.Tally)))
+        _lib.check(self._ctx, rc, "ismc_simulate_batch")
+        return outs["sum"], outs["sumsq"], outs["n_packets"]
 
     @property
     def tuner(self):

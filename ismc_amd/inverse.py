@@ -99,19 +99,13 @@ class LogCost(M.BatchCost):
     LOG_BOX get PENALTY plus their distance to it."""
 
     def __call__(self, points, ids):
-        out = np.zeros(len(points))
-        inside, ins_ids, slots = [], [], []
-        for k, (p, i) in enumerate(zip(points, ids)):
-            p = np.asarray(p, dtype=np.float64)
-            over = np.maximum(LOG_BOX[:, 0] - p, 0) + np.maximum(p - LOG_BOX[:, 1], 0)
-            if not np.all(np.isfinite(p)) or np.any(over > 0):
-                out[k] = M.PENALTY + float(np.sum(np.nan_to_num(over, nan=1e6)))
-            else:
-                inside.append(np.exp(p))
-                ins_ids.append(i)
-                slots.append(k)
-        if inside:
-            out[slots] = super().__call__(inside, ins_ids)
+        Y = np.asarray(points, dtype=np.float64).reshape(len(points), 2)
+        over = np.maximum(LOG_BOX[:, 0] - Y, 0) + np.maximum(Y - LOG_BOX[:, 1], 0)
+        inside = np.all(np.isfinite(Y), axis=1) & ~np.any(over > 0, axis=1)
+        out = np.empty(len(points))
+        out[~inside] = M.PENALTY + np.sum(np.nan_to_num(over[~inside], nan=1e6), axis=1)
+        if inside.any():
+            out[inside] = super().__call__(np.exp(Y[inside]), np.asarray(ids)[inside])
         return out
 
 
@@ -261,6 +255,7 @@ def _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng,
     t_chk = time.perf_counter() - t0 - t_ref
     n_miss = int(miss.sum())
     fb_time = 0.0
+    n_repaired = 0
     if n_miss:
         # MC minimization from the refined point: the misses are mostly
         # measurements whose best fit is poorly conditioned (or not exact
@@ -273,7 +268,12 @@ def _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng,
                                 stream_base=stream_base + 10_000_000, log=log)
         sol[idx] = fb
         fb_time = time.perf_counter() - tf
+        # judge the fallback's results too (reported, not timed)
+        miss2, _, _, _ = check([probs[i] for i in idx], fb, check_packets, seed=11, engine=eng,
+                               refine_packets=refine_packets)
+        n_repaired = int((~miss2).sum())
     return sol, {"refine_s": t_ref, "check_s": t_chk, "fallback_s": fb_time, "misses": n_miss,
+                 "misses_repaired_by_fallback": n_repaired,
                  "mean_evals": float(np.mean(nev)), "launches": st["launches"],
                  "refine_packets": st["packets"], "check_packets": chk_packets,
                  "misfit_R_median": float(np.median(rR)), "misfit_T_median": float(np.median(rT))}

@@ -51,6 +51,9 @@ def test_graph_replay_is_bitwise_identical(engine):
     for rep in range(4):
         g = engine.simulate_batch(cfgs, 30000, seed=17, use_graph=True)
         assert [x.sum for x in g] == [x.sum for x in eager]
+        if rep == 0:
+            # a shape's first call runs eagerly; it is captured when it comes back
+            assert engine.graph_stats() == before
         # unrelated eager work between replays
         engine.simulate(single_layer(2.0, 50.0), 10000, seed=rep)
     # a replay with new configurations of the same shape reads them from staging
@@ -60,7 +63,7 @@ def test_graph_replay_is_bitwise_identical(engine):
     assert [x.sum for x in g2] == [x.sum for x in e2]
     after = engine.graph_stats()
     assert after["captures"] - before["captures"] == 1
-    assert after["launches"] - before["launches"] == 5
+    assert after["launches"] - before["launches"] == 4
 
 
 def test_adaptive_sweep_reaches_target_rse(engine):

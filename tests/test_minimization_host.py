@@ -29,6 +29,16 @@ class AnalyticEngine:
         return {"A": 1 - Rd - Td, "R_d": Rd if cfg.R.present or not cfg.T.present else Rd,
                 "R_s": 0.03, "T_d": Td, "T_u": 0.001 * math.exp(-a)}
 
+    def simulate_batch_array(self, cfg, n, seed=0, streams=None, use_graph=False, **kw):
+        self.calls.append(len(cfg))
+        a, mus, g = cfg["mu_a"][:, 0], cfg["mu_s"][:, 0], cfg["g"][:, 0]
+        s = mus * (1 - g)
+        Rd = 0.5 * s / (s + 8 * a + 5)
+        Td = 0.6 * np.exp(-0.08 * a - 0.01 * s)
+        tal = np.stack([1 - Rd - Td, Rd, np.full_like(Rd, 0.03), Td, 0.001 * np.exp(-a)], 1)
+        npk = np.full(len(cfg), int(n), dtype=np.uint64)
+        return tal * n, tal * n, npk
+
     def simulate_batch(self, cfgs, n, seed=0, streams=None, use_graph=False, **kw):
         self.calls.append(len(cfgs))
         out = []
