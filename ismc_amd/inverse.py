@@ -60,22 +60,50 @@ def make_measurements(count, packets=1_000_000, noise=0.01, seed=0, engine: Opti
             "kernel_ms": res[0].kernel_ms}
 
 
-def surrogate_seeds(sur, R_d, T_d, g=G, n=N_INDEX, t=THICK, grid=256, polish=False):
-    """Initial (mu_a, mu_s') for each measurement from the surrogate."""
+def surrogate_seeds(sur, R_d, T_d, g=G, n=N_INDEX, t=THICK, grid=128, fine=16, polish=False):
+    """Initial (mu_a, mu_s') for each measurement from the surrogate: the
+    best of a grid x grid log grid over mu_a 0.01..10 /cm, mu_s 1..1000 /cm
+    (every measurement against every cell: a global search, so a seed lands
+    on the branch of (R_d, T_d) nearest the measurement), then the best of a
+    fine x fine grid spanning the neighbouring cells around it; optionally
+    polished by PRAXIS on the surrogate."""
+    R_d = np.asarray(R_d, dtype=np.float64)
+    T_d = np.asarray(T_d, dtype=np.float64)
+    Tf = np.maximum(T_d, 1e-6)
+
+    def best_of(cand_a, cand_s, rows):
+        # cand_*: [m, c] log10 coordinates of c candidates for each of m rows
+        pr = sur.predict(10 ** cand_a.ravel(), 10 ** cand_s.ravel(), g, n, t)
+        pR = pr["R_d"].reshape(cand_a.shape)
+        pT = pr["T_d"].reshape(cand_a.shape)
+        c = ((pR - R_d[rows, None]) / R_d[rows, None]) ** 2 + ((pT - T_d[rows, None]) / Tf[rows, None]) ** 2
+        k = np.argmin(c, axis=1)
+        r = np.arange(len(rows))
+        return cand_a[r, k], cand_s[r, k]
+
     la = np.linspace(-2, 1, grid)
     ls = np.linspace(0, 3, grid)
     A, Sg = np.meshgrid(la, ls, indexing="ij")
     pred = sur.predict(10 ** A.ravel(), 10 ** Sg.ravel(), g, n, t)
     pR, pT = pred["R_d"], pred["T_d"]
-    R_d = np.asarray(R_d)
-    T_d = np.asarray(T_d)
-    best = np.zeros(R_d.size, dtype=np.int64)
+    ba = np.zeros(R_d.size)
+    bs = np.zeros(R_d.size)
     for s in range(0, R_d.size, 256):
         r = R_d[s:s + 256, None]
         tt = T_d[s:s + 256, None]
-        c = ((pR[None, :] - r) / r) ** 2 + ((pT[None, :] - tt) / np.maximum(tt, 1e-6)) ** 2
-        best[s:s + 256] = np.argmin(c, axis=1)
-    x0 = np.stack([A.ravel()[best], Sg.ravel()[best]], 1) * math.log(10)   # ln mu_a, ln mu_s
+        c = ((pR[None, :] - r) / r) ** 2 + ((pT[None, :] - tt) / Tf[s:s + 256, None]) ** 2
+        k = np.argmin(c, axis=1)
+        ba[s:s + 256], bs[s:s + 256] = A.ravel()[k], Sg.ravel()[k]
+    if fine > 1:
+        ha, hs = la[1] - la[0], ls[1] - ls[0]
+        off = np.linspace(-1, 1, fine)
+        Oa, Os = np.meshgrid(off, off, indexing="ij")
+        for s in range(0, R_d.size, 1024):
+            rows = np.arange(s, min(s + 1024, R_d.size))
+            ca = np.clip(ba[rows, None] + ha * Oa.ravel()[None, :], -2, 1)
+            cs = np.clip(bs[rows, None] + hs * Os.ravel()[None, :], 0, 3)
+            ba[rows], bs[rows] = best_of(ca, cs, rows)
+    x0 = np.stack([ba, bs], 1) * math.log(10)   # ln mu_a, ln mu_s
     if polish:
         def ev(points, ids):
             P = np.array(points)
@@ -307,7 +335,7 @@ def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_p
     cost_truth = fit_cost(probs, np.stack([meas["mu_a"], meas["mus_reduced"]], 1), check_packets, eng)
     for k, method in enumerate(methods):
         t0 = time.perf_counter()
-        seeds = surrogate_seeds(sur, meas["R_d"], meas["T_d"], polish=(method == "praxis"))
+        seeds = surrogate_seeds(sur, meas["R_d"], meas["T_d"])
         t_seed = time.perf_counter() - t0
         log("seeded refinement: %s (seeds %.1f s)" % (method, t_seed))
         sol, info = _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng, log,
