@@ -176,21 +176,27 @@ def inverse_bench(eng, count):
     sur = Surrogate.load(packaged_weights())
     log = lambda *x: print(*x, file=sys.stderr, flush=True)
     out = {}
-    for key, kw in (("noise_1pct", dict(count=count, plain_count=8, noise=0.01)),
-                    ("noise_0.1pct", dict(count=max(count // 4, 64), plain_count=32, noise=0.001,
-                                          meas_packets=10_000_000, refine_packets=1_000_000,
-                                          check_packets=10_000_000, plain_packets=10_000_000,
-                                          plain_tol=1e-3))):
-        r = inverse.run(sur, engine=eng, methods=("newton",), with_ablation=False, log=log, **kw)
-        n, p = r["seeded_newton"], r["plain"]
-        out[key] = {"measurements": r["count"], "noise": r["noise"], "solves_per_s": n["solves_per_s"],
-                    "plain_solves_per_s": p["solves_per_s"], "plain_solves": p["count"],
-                    "speedup_vs_plain": r["speedup_newton_vs_plain"],
-                    "fits_as_well_as_truth": n["fits_as_well_as_truth"],
-                    "plain_fits_as_well_as_truth": p["fits_as_well_as_truth"],
-                    "errors": n["errors"], "plain_errors": p["errors"], "misses": n["misses"],
-                    "packets": {"measurement": r["meas_packets"], "refinement": r["refine_packets"],
-                                "judge": r["check_packets"], "plain_per_evaluation": r["plain_packets"]}}
+    for key, methods, kw in (
+            ("noise_1pct", ("newton", "praxis"), dict(count=count, plain_count=8, noise=0.01)),
+            ("noise_0.1pct", ("newton",), dict(count=max(count // 4, 64), plain_count=32, noise=0.001,
+                                                meas_packets=10_000_000, refine_packets=1_000_000,
+                                                check_packets=10_000_000, plain_packets=10_000_000,
+                                                plain_tol=1e-3))):
+        r = inverse.run(sur, engine=eng, methods=methods, with_ablation=False, log=log, **kw)
+        p = r["plain"]
+        entry = {"measurements": r["count"], "noise": r["noise"],
+                 "plain_solves_per_s": p["solves_per_s"], "plain_solves": p["count"],
+                 "plain_fits_as_well_as_truth": p["fits_as_well_as_truth"], "plain_errors": p["errors"],
+                 "packets": {"measurement": r["meas_packets"], "refinement": r["refine_packets"],
+                             "judge": r["check_packets"], "plain_per_evaluation": r["plain_packets"]}}
+        for m in methods:
+            n = r["seeded_" + m]
+            entry[{"newton": "defect_correction", "praxis": "praxis"}[m]] = {
+                "solves_per_s": n["solves_per_s"], "speedup_vs_plain": r["speedup_%s_vs_plain" % m],
+                "fits_as_well_as_truth": n["fits_as_well_as_truth"], "errors": n["errors"],
+                "misses": n["misses"], "misses_repaired_by_fallback": n["misses_repaired_by_fallback"],
+                "mc_evaluations_per_solve": n["mean_evals"]}
+        out[key] = entry
     print("bench: timing the CPU inverse baseline", file=sys.stderr, flush=True)
     out["cpu_baseline"] = cpu_inverse_baseline()
     return out
