@@ -21,9 +21,14 @@ sys.path.insert(0, ROOT)
 # transcendental = 4, one Philox4x32-10 uniform = 37):
 #   interaction 174, interface hit 87, entrance 100, sphere decision 40,
 #   re-injection 130, packet set-up 10.
+# Peak: a SIMD issues a full-rate FP32 operation for 32 lanes per cycle (a
+# wave64 instruction in 2 cycles, measured with ismc_issue_rate), so
+# 4 SIMDs x 32 lanes per CU per cycle: 256 CUs x 128 x 2.4 GHz = 78.6 Tslot/s,
+# MI355X's published FP32 vector peak of 157.3 TFLOP/s with an FMA as 2.
 SLOTS = {"interactions": 174, "boundary_hits": 87, "entrances": 100,
          "sphere_decisions": 40, "reinjections": 130}
 SLOTS_PER_PACKET = 10
+LANES_PER_CU_CYCLE = 4 * 32
 
 
 def slots_per_packet(events, n):
@@ -253,7 +258,7 @@ def main():
     W = slots_per_packet(events, N)
     avg_s = sum(kernel_ms) / len(kernel_ms) / 1e3
     achieved = W * N / avg_s / 1e12
-    peak = info["n_cu"] * 64 * info["clock_khz"] * 1e3 / 1e12
+    peak = info["n_cu"] * LANES_PER_CU_CYCLE * info["clock_khz"] * 1e3 / 1e12
     out = {
         "metric": "photon packets/s on 1 GPU",
         "value": value,

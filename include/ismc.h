@@ -207,6 +207,20 @@ int ismc_table_uncertainty(int device, const double* mua, const double* mus, con
                            int32_t M, uint64_t seed, double* mua_unc, double* mus_unc);
 const char* ismc_table_last_error(void);
 
+/* ---------------------------------------------------------------------
+ * VALU issue costs (no counterpart in the reference: the measurement
+ * behind the transport kernel's roofline).  Op k (0 .. ismc_issue_rate_ops()
+ * - 1, named by ismc_issue_rate_name) runs one instruction kind in 8
+ * independent chains per lane, 64 workgroups of 256 lanes per CU, five
+ * launches after a warm-up; returns wave64 instructions per second and the
+ * shader clock the loops ran at (s_memtime over the 100 MHz real-time
+ * counter).  cycles per instruction = 4 x CUs x clock / rate.
+ * --------------------------------------------------------------------- */
+int32_t ismc_issue_rate_ops(void);
+const char* ismc_issue_rate_name(int32_t op);
+int ismc_issue_rate(int device, int32_t op, double* wave_insts_per_s, double* shader_ghz);
+const char* ismc_issue_rate_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

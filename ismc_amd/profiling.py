@@ -14,6 +14,12 @@ activity counters per packet.
 
     python -m ismc_amd.profiling --packets 268435456 --copy-bytes 1073741824 \\
         --trace KT --fetch F --write W --pmc A --pmc B > counters.json
+
+`--issue-rates` instead measures the device's VALU issue cost per wave64
+instruction for single instruction kinds (ismc_issue_rate) and prints them
+as JSON: the per-cycle figures behind the transport kernel's roofline.
+
+    python -m ismc_amd.profiling --issue-rates
 """
 import argparse
 import csv
@@ -101,9 +107,35 @@ def summarize(packets, copy_bytes, trace=None, fetch=None, write=None, pmc=(), k
     return out
 
 
+def issue_rates(device=0):
+    """Wave64 instructions per second, shader clock and cycles per
+    instruction for each instruction kind of ismc_issue_rate."""
+    import ctypes
+
+    from . import _lib
+    from .engine import Engine
+    L = _lib.load()
+    n_cu = Engine(device).device_info()["n_cu"]
+    out = []
+    for k in range(L.ismc_issue_rate_ops()):
+        rate, ghz = ctypes.c_double(), ctypes.c_double()
+        rc = L.ismc_issue_rate(int(device), k, ctypes.byref(rate), ctypes.byref(ghz))
+        if rc != 0:
+            raise RuntimeError("ismc_issue_rate(%d): %s" % (k, L.ismc_issue_rate_last_error().decode()))
+        simds = 4 * n_cu
+        out.append({"op": L.ismc_issue_rate_name(k).decode(), "wave_insts_per_s": rate.value,
+                    "shader_ghz": ghz.value,
+                    "cycles_per_wave_inst": simds * ghz.value * 1e9 / rate.value,
+                    "lanes_per_simd_cycle": 64.0 * rate.value / (simds * ghz.value * 1e9)})
+    return {"n_cu": n_cu, "simds": 4 * n_cu, "ops": out}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--packets", type=int, required=True, help="packets per profiled launch")
+    ap.add_argument("--issue-rates", action="store_true",
+                    help="measure VALU issue costs per instruction kind instead")
+    ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--packets", type=int, help="packets per profiled launch")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--trace")
     ap.add_argument("--fetch")
@@ -111,6 +143,12 @@ def main(argv=None):
     ap.add_argument("--pmc", action="append", default=[])
     ap.add_argument("--kernel", default=KERNEL)
     a = ap.parse_args(argv)
+    if a.issue_rates:
+        json.dump(issue_rates(a.device), sys.stdout, indent=1)
+        print()
+        return
+    if a.packets is None:
+        ap.error("--packets is required")
     json.dump(summarize(a.packets, a.copy_bytes, a.trace, a.fetch, a.write, a.pmc, a.kernel),
               sys.stdout, indent=1)
     print()
