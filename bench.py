@@ -200,7 +200,8 @@ def inverse_bench(eng, count):
                 "solves_per_s": n["solves_per_s"], "speedup_vs_plain": r["speedup_%s_vs_plain" % m],
                 "fits_as_well_as_truth": n["fits_as_well_as_truth"], "errors": n["errors"],
                 "misses": n["misses"], "misses_repaired_by_fallback": n["misses_repaired_by_fallback"],
-                "mc_evaluations_per_solve": n["mean_evals"]}
+                "fallback": n["fallback"], "mc_evaluations_per_solve": n["mean_evals"],
+                "seconds": {k: n[k + "_s"] for k in ("seed", "refine", "check", "fallback")}}
         out[key] = entry
     print("bench: timing the CPU inverse baseline", file=sys.stderr, flush=True)
     out["cpu_baseline"] = cpu_inverse_baseline()

@@ -4,15 +4,19 @@ Pipeline for many measurements (R_d, T_d of a single-layer sample with
 known g, n, t between the standard spheres):
 
 1. seed      the surrogate is evaluated once on a dense (mu_a, mu_s) grid at
-             the sample's (g, n, t); each measurement takes the grid point of
-             least relative misfit, polished by PRAXIS on the surrogate itself
-             (all measurements advance together, one network batch per round);
-2. refine    PRAXIS on the Monte Carlo cost in (ln mu_a, ln mu_s') from the
-             seed, every round of every solve in one transport launch, each
-             solve on its own Philox stream (common random numbers);
+             the sample's (g, n, t), then on a fine local grid around each
+             measurement's best point; each measurement takes the grid point
+             of least relative misfit;
+2. refine    defect correction (`refine_newton`): Gauss-Newton steps on the
+             Monte Carlo residuals with the surrogate's Jacobian, every
+             solve's MC in one transport launch per iteration, each solve on
+             its own Philox stream (common random numbers); or PRAXIS on the
+             MC cost from the seed (`refine_batch`);
 3. judge     one independent MC run per solution at `check_packets`; a
              solution whose R_d or T_d misses the measurement by more than
-             `miss_tol` (relative) is re-solved by plain MC minimization.
+             the combined MC error is re-solved from where it stands by
+             Gauss-Newton on the MC residuals alone, the Jacobian from MC
+             differences (`refine_mc_newton`, the fallback).
 
 The plain method (`solve_plain`) is the reference's: sequential PRAXIS in
 (mu_a, mu_s') with one MC launch per cost evaluation
@@ -28,7 +32,7 @@ import numpy as np
 from . import minimization as M
 from . import praxis as _praxis
 from .config import mus_from_reduced
-from .engine import Engine, default_engine
+from .engine import TALLIES, Engine, default_engine
 from .workloads import standard_sphere
 
 G, N_INDEX, THICK = 0.8, 1.40, 0.1
@@ -37,6 +41,28 @@ G, N_INDEX, THICK = 0.8, 1.40, 0.1
 def problem(R_d, T_d, g=G, n=N_INDEX, t=THICK):
     return M.Problem(n=n, g=g, t=t, R=standard_sphere(), T=standard_sphere(),
                      measured=M.Measured(R_d=float(R_d), T_d=float(T_d)))
+
+
+_RT = [TALLIES.index("R_d"), TALLIES.index("T_d")]
+
+
+def _last_ms(eng):
+    last = getattr(eng, "last_launch", None)
+    return float(last()["kernel_ms"]) if last is not None else 0.0
+
+
+def mc_rt(eng, problems, mua, musr, packets, seed, streams, ids=None, cache=None, use_graph=True):
+    """Mean and standard error ([k, 2] each, columns R_d, T_d) of each
+    problem's set-up at (mu_a, mu_s'), or of problems[ids]: one launch, the
+    configurations patched into cached C templates (the same bytes as
+    problem.configs()), the standard error as Result.se computes it."""
+    cfg = M.config_array(problems, mua, musr, ids=ids, _cache=cache)
+    sums, sq, npk = eng.simulate_batch_array(cfg, packets, seed=seed, streams=streams, use_graph=use_graph)
+    n = npk.astype(np.float64)[:, None]
+    mean = sums / n
+    var = np.maximum(sq / n - mean ** 2, 0.0)
+    se = np.sqrt(var / np.maximum(n - 1, 1))
+    return mean[:, _RT], se[:, _RT]
 
 
 def make_measurements(count, packets=1_000_000, noise=0.01, seed=0, engine: Optional[Engine] = None,
@@ -183,14 +209,13 @@ def refine_newton(problems, seeds, sur, iters=6, packets=100_000, seed=7, engine
     # difference step well above the BF16 network's output resolution (~4e-3)
     h = 0.05
     lo, hi = np.array([[-9.2, -6.9]]), np.array([[6.9, 9.2]])
+    cache = {}
     for it in range(iters):
         mua, mus = np.exp(y_try[:, 0]), np.exp(y_try[:, 1])
-        cfgs = [p.configs(float(a), float(s * (1 - g)))[0][0] for p, a, s in zip(problems, mua, mus)]
-        res = eng.simulate_batch(cfgs, packets, seed=seed, streams=streams, use_graph=True)
+        F, _ = mc_rt(eng, problems, mua, mus * (1 - g), packets, seed, streams, cache=cache)
         stats["launches"] += 1
         stats["packets"] += packets * P
-        stats["kernel_ms"] += res[0].kernel_ms
-        F = np.array([[r.mean["R_d"], r.mean["T_d"]] for r in res])
+        stats["kernel_ms"] += _last_ms(eng)
         r = (F - Mv) / Mv
         better = np.linalg.norm(r, axis=1) < np.linalg.norm(r_acc, axis=1)
         if it > 0:
@@ -218,6 +243,79 @@ def refine_newton(problems, seeds, sur, iters=6, packets=100_000, seed=7, engine
     return sol, np.full(P, iters), stats
 
 
+def refine_mc_newton(problems, start, iters=8, packets=100_000, seed=7, engine: Optional[Engine] = None,
+                     stream_base=0, h=0.05, radius=0.5, s_cut=1e-3, step_tol=1e-3, log=None):
+    """Gauss-Newton on the MC residuals alone, for solves the surrogate
+    cannot guide (the fallback after the judge).  y = (ln mu_a, ln mu_s'),
+    r = (MC(y) - M) / M the relative residuals of R_d and T_d.  Each
+    iteration is one launch that runs, for every pending solve, its trial
+    point y and the points y + h e_1, y + h e_2 on the solve's own Philox
+    stream: r and the forward-difference Jacobian J come from common random
+    numbers (runs that share their histories differ smoothly in y).  A
+    trial whose residual norm is below the accepted one's is accepted with
+    its J and the trust radius doubles; otherwise the radius shrinks
+    fourfold and the next step starts again from the accepted point and its
+    J.  The step is -J^+ r with singular values of J below `s_cut` dropped
+    (far below the surrogate path's cut: an MC difference has no network
+    resolution limit, a weakly resolved direction such as mu_a of a
+    scattering-dominated sample is still worth following, and the
+    acceptance test rejects a step that noise spoils), cut to the trust
+    radius and kept inside LOG_BOX.  A solve stops when its
+    step is shorter than `step_tol` or its radius falls below it.  Returns
+    (best solutions [P, 2] as (mu_a, mu_s'), MC evaluations per solve,
+    stats)."""
+    eng = engine or default_engine()
+    P = len(problems)
+    Mv = np.maximum(np.array([[p.measured.R_d, p.measured.T_d] for p in problems], dtype=np.float64), 1e-6)
+    y_acc = np.log(np.asarray(start, dtype=np.float64).reshape(P, 2))
+    y_try = y_acc.copy()
+    r_acc = np.full((P, 2), np.inf)
+    J_acc = np.zeros((P, 2, 2))
+    rad = np.full(P, float(radius))
+    pending = np.ones(P, dtype=bool)
+    nev = np.zeros(P, dtype=np.int64)
+    streams = (np.arange(P) + stream_base).astype(np.uint32)
+    lo, hi = LOG_BOX[:, 0], LOG_BOX[:, 1] - h
+    cache = {}
+    stats = {"launches": 0, "packets": 0, "kernel_ms": 0.0, "rejected": 0}
+    for it in range(iters):
+        idx = np.where(pending)[0]
+        if len(idx) == 0:
+            break
+        k = len(idx)
+        yt = y_try[idx]
+        pts = np.concatenate([yt, yt + [h, 0.0], yt + [0.0, h]])
+        F, _ = mc_rt(eng, problems, np.exp(pts[:, 0]), np.exp(pts[:, 1]), packets, seed,
+                     np.tile(streams[idx], 3), ids=np.tile(idx, 3), cache=cache)
+        stats["launches"] += 1
+        stats["packets"] += 3 * k * packets
+        stats["kernel_ms"] += _last_ms(eng)
+        nev[idx] += 3
+        F = F.reshape(3, k, 2)
+        Mk = Mv[idx]
+        r = (F[0] - Mk) / Mk
+        J = np.stack([(F[1] - F[0]) / h, (F[2] - F[0]) / h], 2) / Mk[:, :, None]
+        better = np.linalg.norm(r, axis=1) < np.linalg.norm(r_acc[idx], axis=1)
+        stats["rejected"] += int((~better).sum()) if it > 0 else 0
+        acc = idx[better]
+        y_acc[acc] = yt[better]
+        r_acc[acc] = r[better]
+        J_acc[acc] = J[better]
+        rad[idx] = np.where(better, np.minimum(rad[idx] * 2.0, 1.0), rad[idx] * 0.25)
+        U, S, Wt = np.linalg.svd(J_acc[idx])
+        Sinv = np.where(S > s_cut, 1.0 / np.maximum(S, 1e-300), 0.0)
+        dy = -np.einsum("pji,pj,pjk,pk->pi", Wt, Sinv, U, r_acc[idx])
+        nrm = np.linalg.norm(dy, axis=1)
+        dy *= np.minimum(1.0, rad[idx] / np.maximum(nrm, 1e-300))[:, None]
+        y_try[idx] = np.clip(y_acc[idx] + dy, lo, hi)
+        done = (np.linalg.norm(dy, axis=1) < step_tol) | (rad[idx] < step_tol)
+        pending[idx[done]] = False
+        if log:
+            log("    MC Gauss-Newton %d: %d solves, median |r| %.2e, %d rejected, %d done" %
+                (it, k, float(np.median(np.abs(r_acc[idx]).max(1))), int((~better).sum()), int(done.sum())))
+    return np.exp(y_acc), nev, stats
+
+
 def check(problems, sol, packets=1_000_000, seed=9, engine: Optional[Engine] = None,
           refine_packets=100_000, z=4.0, floor=0.002):
     """Independent MC at each solution.  A solution misses when R_d or T_d
@@ -226,20 +324,16 @@ def check(problems, sol, packets=1_000_000, seed=9, engine: Optional[Engine] = N
     refine_packets)) plus a relative floor.  Returns (miss, relative
     misfits R, T, packets)."""
     eng = engine or default_engine()
-    cfgs = [p.configs(float(a), float(s))[0][0] for p, (a, s) in zip(problems, sol)]
-    res = eng.simulate_batch(cfgs, packets, seed=seed, streams=np.arange(len(cfgs), dtype=np.uint32) + 90_000_000)
+    sol = np.asarray(sol, dtype=np.float64)
+    P = len(problems)
+    F, SE = mc_rt(eng, problems, sol[:, 0], sol[:, 1], packets, seed,
+                  np.arange(P, dtype=np.uint32) + 90_000_000, use_graph=False)
+    Mv = np.array([[p.measured.R_d, p.measured.T_d] for p in problems], dtype=np.float64)
     k = math.sqrt(1.0 + packets / float(refine_packets))
-    miss = np.zeros(len(cfgs), dtype=bool)
-    rR = np.zeros(len(cfgs))
-    rT = np.zeros(len(cfgs))
-    for j, (r, p) in enumerate(zip(res, problems)):
-        for key, arr in (("R_d", rR), ("T_d", rT)):
-            meas = getattr(p.measured, key)
-            d = abs(r.mean[key] - meas)
-            arr[j] = d / max(meas, 1e-12)
-            if d > z * k * r.se[key] + floor * meas:
-                miss[j] = True
-    return miss, rR, rT, len(cfgs) * packets
+    d = np.abs(F - Mv)
+    rel = d / np.maximum(Mv, 1e-12)
+    miss = np.any(d > z * k * SE + floor * Mv, axis=1)
+    return miss, rel[:, 0], rel[:, 1], P * packets
 
 
 GENERIC_GUESS = (0.3, 5.0)
@@ -270,7 +364,8 @@ def summary(ea, es):
             "frac_both_within_2pct": float(np.mean((ea <= 0.02) & (es <= 0.02)))}
 
 
-def _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng, log, stream_base):
+def _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng, log, stream_base,
+                  fallback="newton"):
     t0 = time.perf_counter()
     if method == "newton":
         sol, nev, st = refine_newton(probs, seeds, sur, packets=refine_packets, engine=eng,
@@ -289,11 +384,15 @@ def _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng,
         # measurements whose best fit is poorly conditioned (or not exact
         # within the noise), not wrong basins
         idx = np.where(miss)[0]
-        log("  fallback: batched MC minimization from the refined point for %d misses" % n_miss)
+        sub = [probs[i] for i in idx]
+        log("  fallback (%s): MC minimization from the refined point for %d misses" % (fallback, n_miss))
         tf = time.perf_counter()
-        fb, _, _ = refine_batch([probs[i] for i in idx], sol[idx], packets=refine_packets,
-                                step=0.3, tol=3e-3, max_evals=150, engine=eng,
-                                stream_base=stream_base + 10_000_000, log=log)
+        if fallback == "newton":
+            fb, _, _ = refine_mc_newton(sub, sol[idx], packets=refine_packets, engine=eng,
+                                        stream_base=stream_base + 10_000_000, log=log)
+        else:
+            fb, _, _ = refine_batch(sub, sol[idx], packets=refine_packets, step=0.3, tol=3e-3,
+                                    max_evals=150, engine=eng, stream_base=stream_base + 10_000_000, log=log)
         sol[idx] = fb
         fb_time = time.perf_counter() - tf
         # judge the fallback's results too (reported, not timed)
@@ -313,15 +412,16 @@ FIT_TOL = 0.003
 def fit_cost(problems, sol, packets, eng, seed=13):
     """Cost (relative R_d, T_d misfit) of each solution against its measurement,
     MC at `packets` on streams independent of every solver's."""
-    cfgs = [p.configs(float(a), float(s))[0][0] for p, (a, s) in zip(problems, sol)]
-    res = eng.simulate_batch(cfgs, packets, seed=seed, streams=np.arange(len(cfgs), dtype=np.uint32) + 200_000_000)
-    return np.array([M.cost_from({"R_d": r.mean["R_d"], "T_d": r.mean["T_d"]}, p.measured)
-                     for r, p in zip(res, problems)])
+    sol = np.asarray(sol, dtype=np.float64)
+    F, _ = mc_rt(eng, problems, sol[:, 0], sol[:, 1], packets, seed,
+                 np.arange(len(problems), dtype=np.uint32) + 200_000_000, use_graph=False)
+    return np.array([M.cost_from({"R_d": f[0], "T_d": f[1]}, p.measured) for f, p in zip(F, problems)])
 
 
 def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_packets=1_000_000,
         plain_count=8, plain_packets=1_000_000, seed=0, engine: Optional[Engine] = None,
-        methods=("newton", "praxis"), with_ablation=True, log=print, noise=0.01, plain_tol=3e-3):
+        methods=("newton", "praxis"), with_ablation=True, log=print, noise=0.01, plain_tol=3e-3,
+        fallback="newton"):
     """Surrogate-seeded batched solves vs plain MC minimization on the same
     GPU, over `count` synthetic measurements (`noise` relative Gaussian
     noise on R_d and T_d)."""
@@ -339,7 +439,8 @@ def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_p
         t_seed = time.perf_counter() - t0
         log("seeded refinement: %s (seeds %.1f s)" % (method, t_seed))
         sol, info = _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng, log,
-                                  stream_base=k * 40_000_000)
+                                  stream_base=k * 40_000_000, fallback=fallback)
+        info["fallback"] = fallback
         total = t_seed + info["refine_s"] + info["check_s"] + info["fallback_s"]
         info["seed_s"] = t_seed
         info["seconds"] = total

@@ -107,6 +107,45 @@ def _template(problem: Problem):
     return rec
 
 
+def config_array(problems: Sequence["Problem"], mu_a, mus_reduced, ids=None, _cache=None):
+    """The C configurations (CONFIG_DTYPE array) of non-separate problems'
+    evaluations at (mu_a, mu_s'), one per problem (or per entry of `ids`,
+    indices into `problems`, with mu_a and mus_reduced given per entry),
+    patched into the cached templates: byte for byte what
+    problem.configs(mu_a, mu_s')[0][0].to_c() gives, without building the
+    objects.  `_cache` (a dict) keeps the problems' template index between
+    calls on the same problem list."""
+    if _cache is not None and _cache.get("problems") is problems:
+        tpl, index, g = _cache["tpl"], _cache["index"], _cache["g"]
+    else:
+        keys, recs, idx = {}, [], []
+        for p in problems:
+            if p.separate:
+                raise ValueError("config_array takes non-separate problems only")
+            k = p.template_key()
+            if k not in keys:
+                keys[k] = len(recs)
+                recs.append(_template(p))
+            idx.append(keys[k])
+        tpl = np.array(recs, dtype=CONFIG_DTYPE)
+        index = np.array(idx, dtype=np.int64)
+        g = np.array([p.g for p in problems], dtype=np.float64)
+        if _cache is not None:
+            _cache.update(problems=problems, tpl=tpl, index=index, g=g)
+    if ids is not None:
+        ids = np.asarray(ids, dtype=np.int64)
+        index, g = index[ids], g[ids]
+    mu_a = np.asarray(mu_a, dtype=np.float64)
+    musr = np.asarray(mus_reduced, dtype=np.float64)
+    cfg = tpl[index]
+    # mus_from_reduced, elementwise
+    mu_s = np.where(g == 1, musr, musr / np.where(g == 1, 1.0, 1.0 - g))
+    cfg["mu_a"][:, 0] = mu_a
+    cfg["mu_s"][:, 0] = mu_s
+    cfg["mu_t"][:, 0] = mu_a + mu_s
+    return cfg
+
+
 REL_FLOOR = 1e-6
 
 
@@ -156,19 +195,7 @@ class BatchCost:
                                       for p in self.problems], dtype=np.float64)
             self._meas = {q: get(q) for q in ("R_d", "R_s", "T_d", "T_u")}
             self._plain = all(not p.separate for p in self.problems)
-            if self._plain:
-                # each problem's configuration template, as an index into a
-                # small array of the distinct ones
-                keys, recs, index = {}, [], []
-                for p in self.problems:
-                    k = p.template_key()
-                    if k not in keys:
-                        keys[k] = len(recs)
-                        recs.append(_template(p))
-                    index.append(keys[k])
-                self._tpl = np.array(recs, dtype=CONFIG_DTYPE)
-                self._tpl_index = np.array(index, dtype=np.int64)
-                self._g = np.array([p.g for p in self.problems], dtype=np.float64)
+            self._cfg_cache = {}
         return self._meas
 
     def _eval_feasible(self, X, ids):
@@ -178,12 +205,7 @@ class BatchCost:
         as cost_from, so the same values)."""
         ids = np.asarray(ids, dtype=np.int64)
         k = len(ids)
-        cfg = self._tpl[self._tpl_index[ids]]
-        g = self._g[ids]
-        mu_s = np.where(g == 1, X[:, 1], X[:, 1] / np.where(g == 1, 1.0, 1.0 - g))
-        cfg["mu_a"][:, 0] = X[:, 0]
-        cfg["mu_s"][:, 0] = mu_s
-        cfg["mu_t"][:, 0] = X[:, 0] + mu_s
+        cfg = config_array(self.problems, X[:, 0], X[:, 1], ids, self._cfg_cache)
         st = np.asarray(self.streams, dtype=np.int64)[ids] * 4
         sums, _, npk = self.engine.simulate_batch_array(cfg, self.n_packets, seed=self.seed, streams=st,
                                                         use_graph=self.use_graph)

@@ -43,7 +43,8 @@ def test_network_gain(engine, setup):
     assert fit_s > fit_g and fit_s > 0.7
 
 
-def test_judge_flags_misses_and_fallback_repairs_them(engine, setup):
+@pytest.mark.parametrize("fallback", ["mc_newton", "praxis"])
+def test_judge_flags_misses_and_fallback_repairs_them(engine, setup, fallback):
     sur, meas, probs = setup
     k = 32
     sub = probs[:k]
@@ -54,9 +55,15 @@ def test_judge_flags_misses_and_fallback_repairs_them(engine, setup):
     bad[::2] *= [5.0, 1 / 3.0]
     miss, rR, rT, _ = I.check(sub, bad, engine=engine)
     assert miss[::2].all()
-    # the fallback (batched MC PRAXIS from the judged point) brings them back
-    fb, _, _ = I.refine_batch([sub[i] for i in range(0, k, 2)], bad[::2], packets=100_000, step=0.3,
-                              tol=3e-3, max_evals=150, engine=engine, stream_base=3000)
+    # the fallback from the judged point brings them back: Gauss-Newton on
+    # the MC residuals (the pipeline's) or batched MC PRAXIS
+    if fallback == "mc_newton":
+        fb, nev, _ = I.refine_mc_newton([sub[i] for i in range(0, k, 2)], bad[::2], packets=100_000,
+                                        engine=engine, stream_base=3000)
+        assert nev.max() <= 24
+    else:
+        fb, _, _ = I.refine_batch([sub[i] for i in range(0, k, 2)], bad[::2], packets=100_000, step=0.3,
+                                  tol=3e-3, max_evals=150, engine=engine, stream_base=3000)
     before = I.fit_cost([sub[i] for i in range(0, k, 2)], bad[::2], 1_000_000, engine)
     after = I.fit_cost([sub[i] for i in range(0, k, 2)], fb, 1_000_000, engine)
     print("fit cost of the corrupted solutions: median %.3f -> %.4f" % (np.median(before), np.median(after)))

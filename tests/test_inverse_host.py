@@ -82,3 +82,21 @@ def test_defect_correction_rejects_uphill_steps():
                                        engine=eng, s_cut=1e-6)
     assert np.allclose(sol, start)        # the best point evaluated is the start
     assert st["rejected"] == 4
+
+
+def test_mc_newton_fallback_recovers_truth_from_far_starts():
+    """The fallback's Gauss-Newton on (noise-free) residuals with
+    forward-difference Jacobians reaches the truth from starts 2-3x off,
+    launching only the solves still pending."""
+    eng = AnalyticEngine()
+    truths = np.array([(0.05, 2.0), (0.5, 20.0), (3.0, 80.0), (0.2, 150.0)])
+    probs = []
+    for a, s in truths:
+        t = eng.tallies(inverse.problem(0.5, 0.5).configs(a, s)[0][0])
+        probs.append(inverse.problem(t["R_d"], t["T_d"]))
+    start = truths * np.array([[2.5, 0.4], [0.4, 2.0], [1.8, 1.8], [0.5, 0.5]])
+    sol, nev, st = inverse.refine_mc_newton(probs, start, iters=30, packets=1000, engine=eng)
+    assert np.allclose(sol, truths, rtol=2e-3)
+    assert st["launches"] <= 30 and np.all(nev % 3 == 0)
+    # every launch held three points per pending solve, and fewer as solves finished
+    assert all(n % 3 == 0 for n in eng.calls) and eng.calls[-1] <= eng.calls[0]

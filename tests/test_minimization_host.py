@@ -119,3 +119,25 @@ def test_problems_from_ini():
     assert all(p.measured.R_d is not None and p.measured.T_d is not None and p.measured.R_s is None
                for p in probs)
     assert np.std([p.n for p in probs]) > 0
+
+
+def test_config_array_matches_config_objects():
+    """The template-patched C configurations are byte for byte those of
+    Problem.configs(...).to_c(), for several templates and a subset."""
+    from ismc_amd.engine import CONFIG_DTYPE
+    rng = np.random.default_rng(3)
+    S1 = Sphere(True, 0.98, 0.0039, 0.0039, 0.001)
+    S2 = Sphere(True, 0.95, 0.01, 0.002, 0.003)
+    probs = [M.Problem(n=n, g=g, t=t, R=R, T=T, measured=M.Measured(R_d=0.2, T_d=0.3))
+             for n, g, t, R, T in [(1.4, 0.8, 0.1, S1, S1), (1.33, 0.9, 0.2, S2, S1), (1.5, 1.0, 0.05, S1, S2),
+                                   (1.4, 0.8, 0.1, S1, S1), (1.37, 0.0, 0.3, S2, S2)]]
+    a = 10 ** rng.uniform(-2, 1, len(probs))
+    s = 10 ** rng.uniform(0, 2, len(probs))
+    arr = M.config_array(probs, a, s)
+    for i, p in enumerate(probs):
+        ref = np.frombuffer(bytes(p.configs(float(a[i]), float(s[i]))[0][0].to_c()), dtype=CONFIG_DTYPE)[0]
+        assert arr[i].tobytes() == ref.tobytes()
+    ids = np.array([4, 1, 1])
+    sub = M.config_array(probs, a[ids], s[ids], ids=ids)
+    for j, i in enumerate(ids):
+        assert sub[j].tobytes() == arr[i].tobytes()
