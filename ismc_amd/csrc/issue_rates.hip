@@ -12,7 +12,8 @@
 // full-rate FP32 operation (add, FMA, compare, logic) takes about 2 cycles
 // for a 64-lane wave, i.e. 32 lanes per SIMD per cycle without packed
 // math; 32-bit integer multiplies, mad_u64_u32 and FP64 adds take twice
-// that, and transcendentals (rcp, sqrt, log, exp, sin) four times.
+// that, and transcendentals (rcp, sqrt, log, exp, sin) four times.  The
+// three-input v_bitop3_b32 (Philox's key XORs) is measured as well.
 #include <hip/hip_runtime.h>
 
 #include <array>
@@ -32,7 +33,7 @@ constexpr int kLaunches = 5;
 
 enum Op : int {
   ADD_F32, FMA_F32, SHIFT_XOR, MUL_LO_U32, MAD_U64_U32, ADD_F64, RCP_F32, SQRT_F32, LOG_F32,
-  EXP_F32, SIN_F32, N_OPS
+  EXP_F32, SIN_F32, BITOP3, RCP_ADD, MAD64_ADD, BITOP3_ADD, N_OPS
 };
 
 // instruction name and wave instructions per chain step
@@ -41,7 +42,10 @@ constexpr OpInfo kOps[N_OPS] = {
     {"v_add_f32", 1},     {"v_fma_f32", 1},     {"v_lshrrev_b32+v_xor_b32", 2},
     {"v_mul_lo_u32", 1},  {"v_mad_u64_u32", 1}, {"v_add_f64", 1},
     {"v_rcp_f32", 1},     {"v_sqrt_f32", 1},    {"v_log_f32", 1},
-    {"v_exp_f32", 1},     {"v_sin_f32", 1},
+    {"v_exp_f32", 1},     {"v_sin_f32", 1},     {"v_bitop3_b32", 1},
+    // pairs of independent chains: whether the slower kind overlaps the
+    // full-rate add (then a pair costs about the slower one alone)
+    {"v_rcp_f32|v_add_f32", 2}, {"v_mad_u64_u32|v_add_f32", 2}, {"v_bitop3_b32|v_add_f32", 2},
 };
 
 // out: one word per lane; clk: two counts per workgroup.  The grid is
@@ -50,13 +54,14 @@ template <int OP>
 __global__ __launch_bounds__(256) void chain_kernel(uint32_t* out, unsigned long long* clk, int iters,
                                                     uint32_t seed) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  float f[kChains];
+  float f[kChains], f2[kChains];
   uint32_t u[kChains];
   uint64_t w[kChains];
   double d[kChains];
 #pragma unroll
   for (int c = 0; c < kChains; ++c) {
     f[c] = 1.0f + (float)(t + c) * 1e-7f;
+    f2[c] = 2.0f + (float)(t + c) * 1e-7f;
     u[c] = t * 2654435761u + (uint32_t)c + seed;
     w[c] = ((uint64_t)u[c] << 32) | (t + c);
     d[c] = 1.0 + (double)(t + c) * 1e-9;
@@ -78,13 +83,20 @@ __global__ __launch_bounds__(256) void chain_kernel(uint32_t* out, unsigned long
       if constexpr (OP == LOG_F32) f[c] = __builtin_amdgcn_logf(f[c]);
       if constexpr (OP == EXP_F32) f[c] = __builtin_amdgcn_exp2f(f[c]);
       if constexpr (OP == SIN_F32) f[c] = __builtin_amdgcn_sinf(f[c]);
+      if constexpr (OP == BITOP3) u[c] = __builtin_amdgcn_bitop3_b32(u[c], t, seed, 0x96);
+      if constexpr (OP == RCP_ADD) { f[c] = __builtin_amdgcn_rcpf(f[c]); f2[c] = f2[c] + a; }
+      if constexpr (OP == MAD64_ADD) {
+        w[c] = (uint64_t)(uint32_t)w[c] * 0xD2511F53u + w[c];
+        f2[c] = f2[c] + a;
+      }
+      if constexpr (OP == BITOP3_ADD) { u[c] = __builtin_amdgcn_bitop3_b32(u[c], t, seed, 0x96); f2[c] = f2[c] + a; }
     }
   }
   const unsigned long long c1 = __builtin_amdgcn_s_memtime(), r1 = __builtin_amdgcn_s_memrealtime();
   uint32_t r = 0;
 #pragma unroll
   for (int c = 0; c < kChains; ++c)
-    r ^= __float_as_uint(f[c]) ^ u[c] ^ (uint32_t)w[c] ^ (uint32_t)(w[c] >> 32) ^
+    r ^= __float_as_uint(f[c]) ^ __float_as_uint(f2[c]) ^ u[c] ^ (uint32_t)w[c] ^ (uint32_t)(w[c] >> 32) ^
          (uint32_t)__double2loint(d[c]);
   out[t] = r;
   if (threadIdx.x == 0) {

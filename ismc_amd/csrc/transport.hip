@@ -170,14 +170,20 @@ template <> struct DM<double> {
 };
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96 is three-input parity);
+// the compiler does not form it from two XORs by itself
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                        uint32_t k0, uint32_t k1, uint32_t out[4]) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
     const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+    const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
     c1 = (uint32_t)p1;
     c3 = (uint32_t)p0;
     c0 = n0;
@@ -188,10 +194,12 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-// 24-bit uniform in (0, 1]: exact in float and double.
+// 24-bit uniform ((x >> 8) + 1) 2^-24 in (0, 1], exact in float and
+// double: written as one fma of (x >> 8) 2^-24 + 2^-24, whose product and
+// sum are both representable in float, so the fma rounds nothing.
 template <typename R>
 __device__ __forceinline__ R u01(uint32_t x) {
-  return (R)(float)((x >> 8) + 1u) * (R)5.9604644775390625e-08;
+  return (R)fmaf((float)(x >> 8), 5.9604644775390625e-08f, 5.9604644775390625e-08f);
 }
 
 // Unpolarised Fresnel reflectance for light going from index ni to nt at
