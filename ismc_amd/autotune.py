@@ -26,8 +26,8 @@ from typing import Dict, Optional, Sequence, Tuple
 
 import numpy as np
 
-# 0 = the library's default: 4096 packets, fewer for a launch too small to
-# give every resident wave four chunks
+# 0 = the library's default: 256 .. 1024 packets (2048 from 256 points),
+# 14 sqrt(packets per resident wave)
 CHUNKS = (0, 512, 1024, 2048, 4096, 8192, 16384, 65536)
 IDLE = (4, 8, 16, 32)
 BOUND = (4, 8, 16, 32)

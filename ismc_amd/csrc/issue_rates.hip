@@ -33,7 +33,7 @@ constexpr int kLaunches = 5;
 
 enum Op : int {
   ADD_F32, FMA_F32, SHIFT_XOR, MUL_LO_U32, MAD_U64_U32, ADD_F64, RCP_F32, SQRT_F32, LOG_F32,
-  EXP_F32, SIN_F32, BITOP3, RCP_ADD, MAD64_ADD, BITOP3_ADD, N_OPS
+  EXP_F32, SIN_F32, BITOP3, RCP_ADD, MAD64_ADD, BITOP3_ADD, FMA_VGPR, MUL_LITERAL, N_OPS
 };
 
 // instruction name and wave instructions per chain step
@@ -46,6 +46,9 @@ constexpr OpInfo kOps[N_OPS] = {
     // pairs of independent chains: whether the slower kind overlaps the
     // full-rate add (then a pair costs about the slower one alone)
     {"v_rcp_f32|v_add_f32", 2}, {"v_mad_u64_u32|v_add_f32", 2}, {"v_bitop3_b32|v_add_f32", 2},
+    // FMA with every operand in a VGPR, and a multiply by a 32-bit literal:
+    // which of the two makes the literal-addend FMA above slower
+    {"v_fma_f32 (VGPR operands)", 1}, {"v_mul_f32 (literal)", 1},
 };
 
 // out: one word per lane; clk: two counts per workgroup.  The grid is
@@ -67,6 +70,8 @@ __global__ __launch_bounds__(256) void chain_kernel(uint32_t* out, unsigned long
     d[c] = 1.0 + (double)(t + c) * 1e-9;
   }
   const float m = 0.999f + (float)seed * 1e-9f, a = 1e-4f;
+  // per-lane values: kept in VGPRs
+  const float mv = 0.999f + (float)t * 1e-12f, av = 1e-4f + (float)t * 1e-13f;
   const double da = 1e-12 * seed;
   const unsigned long long c0 = __builtin_amdgcn_s_memtime(), r0 = __builtin_amdgcn_s_memrealtime();
   for (int i = 0; i < iters; ++i) {
@@ -90,6 +95,8 @@ __global__ __launch_bounds__(256) void chain_kernel(uint32_t* out, unsigned long
         f2[c] = f2[c] + a;
       }
       if constexpr (OP == BITOP3_ADD) { u[c] = __builtin_amdgcn_bitop3_b32(u[c], t, seed, 0x96); f2[c] = f2[c] + a; }
+      if constexpr (OP == FMA_VGPR) f[c] = fmaf(f[c], mv, av);
+      if constexpr (OP == MUL_LITERAL) f[c] = f[c] * 0.99990004f;
     }
   }
   const unsigned long long c1 = __builtin_amdgcn_s_memtime(), r1 = __builtin_amdgcn_s_memrealtime();

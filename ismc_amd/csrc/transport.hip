@@ -36,8 +36,11 @@
 //      interface:    w0 Fresnel test, w1 return draw, w2 new |mu_z| (on exit)
 //    so a history is a pure function of (seed, stream, packet id):
 //    results do not depend on launch shape or scheduling.
-//  * Work queue.  The host splits every configuration ("point") into chunks
-//    of packet ids.  Each wave takes chunks from a global atomic counter;
+//  * Work queue.  Every configuration ("point") is split into chunks of
+//    packet ids, numbered point after point.  Each wave takes chunk numbers
+//    from a global atomic counter and finds the chunk's point by a binary
+//    search over the points' first chunk numbers (one 32-byte range record
+//    per point, so the host writes and uploads O(points), not O(chunks));
 //    the chunk's point table (layers + sphere constants) is copied into a
 //    per-wave LDS slot (two slots, so lanes can start the next chunk while
 //    others finish the last one).  The current layer's constants live in
@@ -122,19 +125,21 @@ struct alignas(16) Point {
   Layer<R> layer[kMaxLayers];
 };
 
-struct alignas(16) Chunk {
-  uint32_t point;
+// a point's packets in the launch's queue
+struct alignas(16) Range {
+  uint64_t first_pid;  // packet id of the point's first packet
+  uint64_t out_base;   // dense index of that packet (per-packet output)
+  uint64_t n_packets;
+  uint32_t chunk0;     // number of the point's first chunk
   uint32_t stream;
-  uint64_t first_pid;
-  uint32_t count;
-  uint32_t _pad;
-  uint64_t out_base;  // dense packet index of first_pid (per-packet output)
 };
 
 struct Args {
   const void* points;        // Point<R>[n_points]
-  const Chunk* chunks;
+  const Range* ranges;       // [n_points]
+  uint32_t n_points;
   uint32_t n_chunks;
+  uint32_t chunk_packets;
   uint32_t seed_lo, seed_hi;
   uint32_t* chunk_counter;
   unsigned long long* slots; // [n_points][kSlotWords]
@@ -280,10 +285,18 @@ __device__ __forceinline__ void copy_point(Point<R>* dst, const Point<R>* src, i
 template <typename R>
 __device__ __forceinline__ void flush_slot(unsigned long long* acc, unsigned long long* slots,
                                            uint32_t point, int lane) {
-  if (lane < kSlotWords) {
-    const unsigned long long v = acc[lane];
-    if (v) atomicAdd(&slots[(size_t)point * kSlotWords + lane], v);
-    acc[lane] = 0ull;
+  // The point's row address is wave-uniform (scalar registers).  The lane
+  // index passes through an empty asm operand so that the compiler cannot
+  // hoist a 64-bit per-lane address out of the kernel's loop (it would
+  // hold two VGPRs for the whole kernel, or spill them); the asm emits no
+  // instruction.
+  unsigned long long* row = slots + (size_t)__builtin_amdgcn_readfirstlane(point) * kSlotWords;
+  uint32_t l = (uint32_t)lane;
+  asm volatile("" : "+v"(l));
+  if (l < (uint32_t)kSlotWords) {
+    const unsigned long long v = acc[l];
+    if (v) atomicAdd(&row[l], v);
+    acc[l] = 0ull;
   }
 }
 
@@ -424,18 +437,29 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
             if (ns) set_point1 = 0xffffffffu; else set_point0 = 0xffffffffu;
             continue;
           }
-          const Chunk ch = a.chunks[k];
-          copy_point<R>(&pts[ns], &gpts[ch.point], lane);
+          // the chunk's point: the last one whose first chunk is <= k
+          // (points without packets share their successor's first chunk
+          // and are passed over)
+          uint32_t lo = 0, hi = a.n_points;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.ranges[mid].chunk0 <= k) lo = mid; else hi = mid;
+          }
+          const Range rg = a.ranges[lo];
+          const uint64_t off = (uint64_t)(k - rg.chunk0) * a.chunk_packets;
+          const uint64_t cnt = rg.n_packets - off < (uint64_t)a.chunk_packets ? rg.n_packets - off
+                                                                              : (uint64_t)a.chunk_packets;
+          copy_point<R>(&pts[ns], &gpts[lo], lane);
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          if (ns) set_point1 = ch.point; else set_point0 = ch.point;
+          if (ns) set_point1 = lo; else set_point0 = lo;
           cur_set = ns;
-          win_next = ch.first_pid;
-          win_end = ch.first_pid + ch.count;
-          win_first = ch.first_pid;
-          win_outbase = ch.out_base;
-          win_stream = ch.stream;
+          win_next = rg.first_pid + off;
+          win_end = win_next + cnt;
+          win_first = win_next;
+          win_outbase = rg.out_base + off;
+          win_stream = rg.stream;
         }
         const uint64_t avail = win_end - win_next;
         const uint32_t nneed = (uint32_t)__popcll(need);
@@ -512,47 +536,59 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     if (__ballot(run_flight || run_enter) == 0) break;
     uint32_t rw[4];
     philox((uint32_t)pid, (uint32_t)(pid >> 32), ev, stream, k0, k1, rw);
-    const R u0 = u01<R>(rw[0]), u1 = u01<R>(rw[1]), u2 = u01<R>(rw[2]), u3 = u01<R>(rw[3]);
+    // The event's first three uniforms go straight into the registers an
+    // interface hit or a reflected entrance keeps them in (bu0: Fresnel
+    // draw; su1, su2: return draw and new |mu_z|).  Every lane that runs
+    // an event overwrites them; they are read only after a hit or a
+    // reflected entrance, whose own event is the last to write them, and a
+    // lane waiting for that read runs no event meanwhile.
+    if (run_flight || run_enter) {
+      bu0 = u01<R>(rw[0]);
+      su1 = u01<R>(rw[1]);
+      su2 = u01<R>(rw[2]);
+    }
+    const R u0 = bu0, u1 = su1, u2 = su2, u3 = u01<R>(rw[3]);
 
     if (run_flight) {
       const R zb = muz > (R)0 ? L_zhi : L_zlo;
       const R dz = zb - z;
       const bool glass = (L_kind == KIND_GLASS);
-      const bool hit = glass || (muz != (R)0 && s * DM<R>::abs(muz) >= DM<R>::abs(dz));
-      if (hit) {
-        // reaches the interface: record the event's draws for the interface pass
-        bu0 = u0;
-        su1 = u1;
-        su2 = u2;
-        phase = PH_BOUND;
-      } else {
-        // hop, drop, roulette, spin (CUDAISMLMC.h:259-309)
-        z += s * muz;
-        const R dW = w * L_abs;
-        w -= dW;
-        A += dW;
-        e_int += 1;
-        if (w <= P.roul_thr) {
-          const bool survive = !P.cuda_roulette && (u3 <= P.roul_p);
-          w = survive ? w * P.roul_m : (R)0;
-        }
-        if (w > (R)0) {
-          const R ct = hg_cos<R>(L_kind, L_ha, L_hb, L_hc, L_hd, L_he, u1);
-          muz = spin_z<R>(muz, ct, u2);
-          s = DM<R>::nlog2(u0) * L_step;
-        } else {
-          phase = PH_END;
-        }
+      // bitwise, not short-circuit: no branches around the comparisons
+      const bool hit = glass | ((muz != (R)0) & (s * DM<R>::abs(muz) >= DM<R>::abs(dz)));
+      // A hit reaches the interface (its draws wait in bu0, su1, su2);
+      // otherwise hop, drop, roulette, spin (CUDAISMLMC.h:259-309).  The
+      // interaction's new state is computed for every lane in flight (the
+      // wave runs this code whenever one of its lanes interacts) and taken
+      // where the lane interacted, which saves the copies that merging two
+      // branches costs.
+      const R dW = w * L_abs;
+      R w_new = w - dW;
+      if (!hit && w_new <= P.roul_thr) {
+        const bool survive = !P.cuda_roulette && (u3 <= P.roul_p);
+        w_new = survive ? w_new * P.roul_m : (R)0;
       }
+      const R ct = hg_cos<R>(L_kind, L_ha, L_hb, L_hc, L_hd, L_he, u1);
+      const R muz_new = spin_z<R>(muz, ct, u2);
+      const R s_new = DM<R>::nlog2(u0) * L_step;
+      const bool live = w_new > (R)0;
+      if (!hit) {
+        z += s * muz;
+        A += dW;
+        w = w_new;
+        e_int += 1;
+      }
+      if (!hit && live) {
+        muz = muz_new;
+        s = s_new;
+      }
+      phase = hit ? PH_BOUND : (live ? PH_FLIGHT : PH_END);
     } else if (run_enter) {
       // (re-)entrance through the outer face (CUDAISMLMC.h:367-369, 502-503)
       e_inc += 1;
       R ct;
       const R Re = entrance_R<R>(P, side, muz, ct);
       if (u0 < Re) {
-        su1 = u1;
-        su2 = u2;
-        phase = PH_SPHERE;
+        phase = PH_SPHERE;  // su1, su2 hold this event's return draw and new |mu_z|
       } else {
         muz = DM<R>::csign(ct, muz);
         layer = P.first_layer[side];
@@ -651,7 +687,7 @@ struct ismc_ctx {
   hipStream_t stream = nullptr;
   hipDeviceProp_t prop{};
   std::string err;
-  DevBuf points, chunks, slots, counter, per_packet, eval_in, eval_out;
+  DevBuf points, ranges, slots, counter, per_packet, eval_in, eval_out;
   void* h_stage = nullptr;  // pinned staging
   size_t h_stage_cap = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -849,10 +885,20 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   else
     HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, false>, kBlock, 0));
   const int per_cu = (tune && tune->blocks_per_cu > 0) ? bpc : std::max(1, resident);
-  // packets per queue chunk: the caller's, else the default, lowered for a
-  // small launch to give every resident wave at least four chunks (a wave
-  // runs a chunk's packets itself, so a few large chunks of long histories
-  // would leave most of the chip idle while they finish)
+  // packets per queue chunk: the caller's, else a size fitted to the
+  // launch.  Each chunk costs a queue atomic, a point-table copy and a
+  // tally flush (16 atomics on the point's slots, contended when every
+  // wave runs the same point), so small chunks are slow; a wave runs a
+  // chunk's packets itself, so large chunks leave a tail of waves
+  // finishing long histories.  Measured on MI355X (single points of 1e6,
+  // 1e7 and 1e8 packets, 2^26 packets of three layers, 2^28 of the slab
+  // between slides, 4096 points of 1e5), the best size grows like the
+  // square root of the packets per resident wave: 256 at ~140, 512 at
+  // ~1,400, 1024 from ~10^4.  So 14 sqrt(packets per wave), kept within
+  // 256 .. 1024 (and the context's default as a ceiling).  A launch of
+  // many points switches point at almost every chunk, each switch a flush
+  // and a table copy, and does best with 2048 (4096 points of 1e5: 100.6 ms
+  // against 105.1 at 1024), so from 256 points the ceiling is 2048.
   int chunk = c->chunk_packets;
   if (tune && tune->chunk_packets > 0) {
     chunk = tune->chunk_packets;
@@ -860,42 +906,43 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
     uint64_t requested = 0;
     for (int p = 0; p < n_points; ++p) requested += std::min<uint64_t>(npk[p], 1ull << 32);
     const uint64_t waves = (uint64_t)c->prop.multiProcessorCount * per_cu * kWavesPerBlock;
+    const uint64_t fit = 14 * (uint64_t)std::sqrt((double)(requested / waves));
+    const uint64_t ceiling = n_points >= 256 ? 2048 : 1024;
     chunk = (int)std::min<uint64_t>((uint64_t)c->chunk_packets,
-                                    std::max<uint64_t>(64, requested / (4 * waves)));
+                                    std::min<uint64_t>(ceiling, std::max<uint64_t>(256, fit)));
   }
-  // chunk list
-  std::vector<Chunk> chunks;
-  uint64_t total = 0;
+  // the queue: one range record per point; chunk numbers run point after point
+  std::vector<Range> ranges((size_t)n_points);
+  uint64_t total = 0, n_chunks = 0;
   for (int p = 0; p < n_points; ++p) {
     if (npk[p] > (1ull << 32))
       return set_err(c, ISMC_ERANGE, "point %s%lld: more than 2^32 packets in one call", "", p);
     const uint64_t f0 = first ? first[p] : 0ull;
     if (f0 + npk[p] < f0) return set_err(c, ISMC_ERANGE, "packet id range overflows%s");
-    for (uint64_t k = 0; k < npk[p]; k += (uint64_t)chunk) {
-      Chunk ch{};
-      ch.point = (uint32_t)p;
-      ch.stream = streams ? streams[p] : (uint32_t)p;
-      ch.first_pid = f0 + k;
-      ch.count = (uint32_t)std::min<uint64_t>((uint64_t)chunk, npk[p] - k);
-      ch.out_base = total + k;
-      chunks.push_back(ch);
-    }
+    if (n_chunks > 0xffffffffull) return set_err(c, ISMC_ERANGE, "too many chunks%s");
+    Range& r = ranges[(size_t)p];
+    r.first_pid = f0;
+    r.out_base = total;
+    r.n_packets = npk[p];
+    r.chunk0 = (uint32_t)n_chunks;
+    r.stream = streams ? streams[p] : (uint32_t)p;
+    n_chunks += (npk[p] + (uint64_t)chunk - 1) / (uint64_t)chunk;
     total += npk[p];
   }
-  if (chunks.size() > 0xffffffffull) return set_err(c, ISMC_ERANGE, "too many chunks%s");
+  if (n_chunks > 0xffffffffull) return set_err(c, ISMC_ERANGE, "too many chunks%s");
   const size_t pbytes = sizeof(Point<R>) * (size_t)n_points;
-  const size_t cbytes = sizeof(Chunk) * std::max<size_t>(chunks.size(), 1);
+  const size_t cbytes = sizeof(Range) * (size_t)n_points;
   const size_t sbytes = sizeof(unsigned long long) * kSlotWords * (size_t)n_points;
   int rc;
   if ((rc = ensure_dev(c, c->points, pbytes))) return rc;
-  if ((rc = ensure_dev(c, c->chunks, cbytes))) return rc;
+  if ((rc = ensure_dev(c, c->ranges, cbytes))) return rc;
   if ((rc = ensure_dev(c, c->slots, sbytes))) return rc;
   if ((rc = ensure_dev(c, c->counter, 64))) return rc;
   if ((rc = ensure_stage(c, pbytes + cbytes + sbytes))) return rc;
   char* st = (char*)c->h_stage;
   Point<R>* hp = reinterpret_cast<Point<R>*>(st);
   for (int p = 0; p < n_points; ++p) compile_point<R>(&cfgs[p], flags, &hp[p]);
-  memcpy(st + pbytes, chunks.data(), sizeof(Chunk) * chunks.size());
+  memcpy(st + pbytes, ranges.data(), cbytes);
   const size_t nslot = (size_t)kSlotWords * n_points;
   if (per_packet_host) {
     if ((rc = ensure_dev(c, c->per_packet, sizeof(double) * 6 * std::max<uint64_t>(total, 1))))
@@ -904,8 +951,10 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
 
   Args a{};
   a.points = c->points.p;
-  a.chunks = (const Chunk*)c->chunks.p;
-  a.n_chunks = (uint32_t)chunks.size();
+  a.ranges = (const Range*)c->ranges.p;
+  a.n_points = (uint32_t)n_points;
+  a.n_chunks = (uint32_t)n_chunks;
+  a.chunk_packets = (uint32_t)chunk;
   a.seed_lo = (uint32_t)seed;
   a.seed_hi = (uint32_t)(seed >> 32);
   a.chunk_counter = (uint32_t*)c->counter.p;
@@ -921,15 +970,12 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   const int grid = (int)std::max<uint64_t>(
       1, std::min<uint64_t>((uint64_t)c->prop.multiProcessorCount * per_cu, blocks_useful));
   unsigned long long* hs = reinterpret_cast<unsigned long long*>(st + pbytes + cbytes);
-  const size_t cbytes_used = sizeof(Chunk) * chunks.size();
-  // the launch sequence: upload point tables and the chunk queue from pinned
+  // the launch sequence: upload point tables and the range records from pinned
   // staging, clear the tally slots and queue counter with a kernel (no
   // memset nodes), run the transport, download the slots
   auto enqueue = [&](bool timed) -> int {
     HIPCHECK(c, hipMemcpyAsync(c->points.p, st, pbytes, hipMemcpyHostToDevice, c->stream));
-    if (cbytes_used)
-      HIPCHECK(c, hipMemcpyAsync(c->chunks.p, st + pbytes, cbytes_used, hipMemcpyHostToDevice,
-                                 c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->ranges.p, st + pbytes, cbytes, hipMemcpyHostToDevice, c->stream));
     zero_u64_kernel<<<(unsigned)((nslot + 255) / 256), 256, 0, c->stream>>>(
         (unsigned long long*)c->slots.p, nslot, (uint32_t*)c->counter.p);
     HIPCHECK(c, hipGetLastError());
@@ -947,8 +993,8 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   ismc_ctx::Graph* hit = nullptr;
   if (use_graph) {
     const std::vector<uint64_t> key = {
-        sizeof(R), (uint64_t)n_points, (uint64_t)chunks.size(), (uint64_t)grid,
-        (uint64_t)(uintptr_t)c->points.p, (uint64_t)(uintptr_t)c->chunks.p,
+        sizeof(R), (uint64_t)n_points, n_chunks, (uint64_t)chunk, (uint64_t)grid,
+        (uint64_t)(uintptr_t)c->points.p, (uint64_t)(uintptr_t)c->ranges.p,
         (uint64_t)(uintptr_t)c->slots.p, (uint64_t)(uintptr_t)c->counter.p,
         (uint64_t)(uintptr_t)c->h_stage, pbytes, cbytes, sbytes, seed};
     for (auto& g : c->graphs)
@@ -1006,7 +1052,7 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipEventElapsedTime(&c->last_ms, c->e0, c->e1));
   c->last_grid = grid;
-  c->last_chunks = (int64_t)chunks.size();
+  c->last_chunks = (int64_t)n_chunks;
   const double inv = 1.0 / 4294967296.0;
   for (int p = 0; p < n_points; ++p) {
     const unsigned long long* v = hs + (size_t)p * kSlotWords;
@@ -1065,7 +1111,7 @@ void ismc_destroy(ismc_ctx* c) {
   hipError_t e = hipSetDevice(c->device);
   if (c->stream) e = hipStreamSynchronize(c->stream);
   drop_graphs(c);
-  DevBuf* bufs[] = {&c->points, &c->chunks, &c->slots, &c->counter, &c->per_packet, &c->eval_in,
+  DevBuf* bufs[] = {&c->points, &c->ranges, &c->slots, &c->counter, &c->per_packet, &c->eval_in,
                     &c->eval_out};
   for (DevBuf* b : bufs)
     if (b->p) e = hipFree(b->p);

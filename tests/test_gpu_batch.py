@@ -106,3 +106,38 @@ def test_autotuned_launch_is_bitwise_identical(engine):
             assert x.sum == y.sum and x.sumsq == y.sumsq and x.events == y.events
     r = engine.simulate_batch(cfgs, npk, seed=77, autotune=True)
     assert all(x.sum == y.sum for x, y in zip(r, base))
+
+
+def test_many_chunks_per_wave_are_bitwise_identical(engine):
+    """Shapes where every wave runs many small chunks of several points, so
+    its two table slots change hands often: tallies and event counts equal
+    the default launch's bit for bit."""
+    cfgs = [single_layer(a, s) for a in (0.05, 1.0, 8.0) for s in (5.0, 50.0, 400.0)]
+    npk = np.array([20000, 3000, 90000, 1000, 50000, 7000, 40000, 2000, 60000], dtype=np.uint64)
+    base = engine.simulate_batch(cfgs, npk, seed=77)
+    for b, c, i, bl in ((1, 0, 4, 8), (2, 0, 8, 16), (1, 64, 16, 8), (2, 128, 4, 16), (3, 64, 32, 8),
+                        (4, 256, 8, 8)):
+        r = engine.simulate_batch(cfgs, npk, seed=77, blocks_per_cu=b, chunk_packets=c, idle_lanes=i,
+                                  bound_lanes=bl)
+        bad = [k for k, (x, y) in enumerate(zip(r, base)) if x.sum != y.sum or x.events != y.events]
+        assert not bad, "blocks/CU %d, chunk %d, idle %d, bound %d: points %s differ" % (b, c, i, bl, bad)
+
+
+def test_points_without_packets_are_passed_over(engine):
+    """A batch with points of 0 packets at the start, in the middle and at the
+    end: those points tally nothing, and every other point equals a batch
+    of the non-empty points alone (same streams), for single-chunk and
+    many-chunk shapes."""
+    cfgs = [single_layer(a, s) for a in (0.05, 1.0, 8.0) for s in (5.0, 50.0)]
+    npk = np.array([0, 20000, 0, 0, 7000, 3000, 0], dtype=np.uint64)
+    cfg7 = [cfgs[k % len(cfgs)] for k in range(len(npk))]
+    streams = np.arange(len(npk), dtype=np.uint32) + 40
+    nz = np.where(npk > 0)[0]
+    ref = engine.simulate_batch([cfg7[k] for k in nz], npk[nz], seed=5, streams=streams[nz])
+    for chunk in (0, 64, 100000):
+        r = engine.simulate_batch(cfg7, npk, seed=5, streams=streams, chunk_packets=chunk)
+        for k in range(len(npk)):
+            if npk[k] == 0:
+                assert r[k].n_packets == 0 and all(v == 0 for v in r[k].sum.values())
+        for j, k in enumerate(nz):
+            assert r[k].sum == ref[j].sum and r[k].events == ref[j].events, (chunk, k)

@@ -13,7 +13,7 @@ def test_issue_rates(engine):
     r = profiling.issue_rates(engine.device)
     ops = {o["op"]: o for o in r["ops"]}
     print({k: round(v["cycles_per_wave_inst"], 2) for k, v in ops.items()})
-    assert len(ops) == 15
+    assert len(ops) == 17
     for o in r["ops"]:
         assert o["wave_insts_per_s"] > 0
         assert 0.5 < o["shader_ghz"] < 3.5
