@@ -207,6 +207,16 @@ int ismc_table_uncertainty(int device, const double* mua, const double* mus, con
                            int32_t M, uint64_t seed, double* mua_unc, double* mus_unc);
 const char* ismc_table_last_error(void);
 
+/* For each of M measurements (R[m], T[m]) the index of the cell of a
+ * G-cell grid of predicted readings (pR, pT) with the least relative
+ * misfit ((pR - R)/R)^2 + ((pT - T)/max(T, 1e-6))^2: numpy's argmin over
+ * the grid, first index among equal minima, a NaN cost first (no
+ * counterpart in the reference: the global search that seeds the inverse
+ * solver from the surrogate).  FP64, host pointers, blocks; errors in
+ * ismc_table_last_error. */
+int ismc_grid_argmin(int device, const float* pR, const float* pT, int32_t G, const double* R,
+                     const double* T, int32_t M, int32_t* idx);
+
 /* ---------------------------------------------------------------------
  * VALU issue costs (no counterpart in the reference: the measurement
  * behind the transport kernel's roofline).  Op k (0 .. ismc_issue_rate_ops()

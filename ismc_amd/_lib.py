@@ -22,7 +22,7 @@ EXPORTS = ("ismc_create", "ismc_destroy", "ismc_last_error", "ismc_device_info_g
            "ismc_simulate", "ismc_simulate_batch", "ismc_simulate_packets", "ismc_last_launch",
            "ismc_eval_physics", "ismc_validate_config", "ismc_copy_probe", "ismc_graph_stats",
            "ismc_table_uncertainty", "ismc_table_last_error", "ismc_issue_rate_ops",
-           "ismc_issue_rate_name", "ismc_issue_rate", "ismc_issue_rate_last_error")
+           "ismc_issue_rate_name", "ismc_issue_rate", "ismc_issue_rate_last_error", "ismc_grid_argmin")
 
 
 class Sphere(ctypes.Structure):
@@ -113,6 +113,8 @@ def load():
         L.ismc_table_uncertainty.restype = ctypes.c_int
         L.ismc_table_last_error.argtypes = []
         L.ismc_table_last_error.restype = ctypes.c_char_p
+        L.ismc_grid_argmin.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int32, vp]
+        L.ismc_grid_argmin.restype = ctypes.c_int
         L.ismc_issue_rate_ops.argtypes = []
         L.ismc_issue_rate_ops.restype = ctypes.c_int32
         L.ismc_issue_rate_name.argtypes = [ctypes.c_int32]

@@ -573,10 +573,12 @@ int ismc_mlp_get_params(ismc_mlp* p, float* flat) {
 }
 
 int ismc_mlp_forward(ismc_mlp* p, const float* x, int32_t n, float* y) {
-  if (!p || !x || !y) return ISMC_EINVAL;
+  if (!p || !x || !y || n < 0) return ISMC_EINVAL;
   MCHECK(p, hipSetDevice(p->device));
   int rc;
-  std::vector<float> yb((size_t)p->max_batch * p->out_pad);
+  // host buffer for one batch's padded outputs, sized by this call's rows
+  const int32_t rows = std::min<int32_t>(p->max_batch, ((n + TM - 1) / TM) * TM);
+  std::vector<float> yb((size_t)rows * p->out_pad);
   for (int32_t s = 0; s < n; s += p->max_batch) {
     const int cnt = std::min<int32_t>(p->max_batch, n - s);
     const int B = ((cnt + TM - 1) / TM) * TM;

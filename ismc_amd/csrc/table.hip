@@ -133,6 +133,57 @@ __global__ __launch_bounds__(256) void uncertainty_kernel(Args a) {
   a.mus_unc[cell] = 100.0 * sqrt(var_s) / cs;
 }
 
+// For measurement m, the index of the grid cell c minimizing
+//   ((pR[c] - R[m]) / R[m])^2 + ((pT[c] - T[m]) / max(T[m], 1e-6))^2
+// with numpy's argmin rules: the first index among equal minima, and a NaN
+// cost counts as the minimum (the first NaN wins).  FP64, the operations in
+// that order and nothing contracted into an fma, so the costs are the ones
+// numpy computes bit for bit.  One workgroup per measurement; its lanes
+// stride over the cells, then reduce (cost, index) pairs in LDS.
+constexpr int kArgminBlock = 256;
+
+__device__ __forceinline__ bool before(double v, int i, double bv, int bi) {
+  const bool vn = isnan(v), bn = isnan(bv);
+  if (vn != bn) return vn;
+  if (!vn && v != bv) return v < bv;
+  return i < bi;
+}
+
+__global__ __launch_bounds__(kArgminBlock) void grid_argmin_kernel(const float* __restrict__ pR,
+                                                                   const float* __restrict__ pT, int G,
+                                                                   const double* __restrict__ R,
+                                                                   const double* __restrict__ T,
+                                                                   int32_t* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double s_v[kArgminBlock];
+  __shared__ int s_i[kArgminBlock];
+  const int m = blockIdx.x;
+  const double r = R[m], t = T[m], tf = t > 1e-6 ? t : 1e-6;
+  double bv = 0.0;
+  int bi = 0x7fffffff;
+  for (int c = threadIdx.x; c < G; c += kArgminBlock) {
+    const double a = ((double)pR[c] - r) / r;
+    const double b = ((double)pT[c] - t) / tf;
+    const double v = a * a + b * b;
+    if (bi == 0x7fffffff || before(v, c, bv, bi)) { bv = v; bi = c; }
+  }
+  s_v[threadIdx.x] = bv;
+  s_i[threadIdx.x] = bi;
+  __syncthreads();
+  for (int h = kArgminBlock / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      const int j = threadIdx.x + h;
+      if (s_i[j] != 0x7fffffff && (s_i[threadIdx.x] == 0x7fffffff ||
+                                   before(s_v[j], s_i[j], s_v[threadIdx.x], s_i[threadIdx.x]))) {
+        s_v[threadIdx.x] = s_v[j];
+        s_i[threadIdx.x] = s_i[j];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[m] = s_i[0];
+}
+
 thread_local std::string g_err;
 
 int fail(int code, const char* what, hipError_t e) {
@@ -189,6 +240,50 @@ int ismc_table_uncertainty(int device, const double* mua, const double* mus, con
   if ((e = hipGetLastError()) != hipSuccess) { rc = fail(ISMC_EDEVICE, "uncertainty_kernel", e); goto out; }
   if ((e = hipMemcpyAsync(mua_unc, a.mua_unc, tb, hipMemcpyDeviceToHost, s)) != hipSuccess ||
       (e = hipMemcpyAsync(mus_unc, a.mus_unc, tb, hipMemcpyDeviceToHost, s)) != hipSuccess) {
+    rc = fail(ISMC_EDEVICE, "hipMemcpyAsync", e);
+    goto out;
+  }
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) rc = fail(ISMC_EDEVICE, "hipStreamSynchronize", e);
+out:
+  if (s) (void)hipStreamDestroy(s);
+  (void)hipFree(d);
+  return rc;
+}
+
+int ismc_grid_argmin(int device, const float* pR, const float* pT, int32_t G, const double* R,
+                     const double* T, int32_t M, int32_t* idx) {
+  using namespace tablek;
+  if (!pR || !pT || !R || !T || !idx || G < 1 || M < 0) {
+    g_err = "ismc_grid_argmin: null pointer, G < 1 or M < 0";
+    return ISMC_EINVAL;
+  }
+  if (M == 0) return 0;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(ISMC_EDEVICE, "hipSetDevice", e);
+  const size_t gb = (size_t)G * sizeof(float), mb = (size_t)M * sizeof(double), ib = (size_t)M * sizeof(int32_t);
+  char* d = nullptr;
+  e = hipMalloc(&d, 2 * gb + 2 * mb + ib + 64);
+  if (e != hipSuccess) return fail(ISMC_ENOMEM, "hipMalloc", e);
+  // 8-byte aligned: the two double arrays first
+  double* dR = (double*)d;
+  double* dT = dR + M;
+  float* dpR = (float*)(dT + M);
+  float* dpT = dpR + G;
+  int32_t* dI = (int32_t*)(dpT + G);
+  int rc = 0;
+  hipStream_t s = nullptr;
+  const void* src[4] = {R, T, pR, pT};
+  void* dst[4] = {dR, dT, dpR, dpT};
+  const size_t sz[4] = {mb, mb, gb, gb};
+  if ((e = hipStreamCreate(&s)) != hipSuccess) { rc = fail(ISMC_EDEVICE, "hipStreamCreate", e); goto out; }
+  for (int k = 0; k < 4; ++k)
+    if ((e = hipMemcpyAsync(dst[k], src[k], sz[k], hipMemcpyHostToDevice, s)) != hipSuccess) {
+      rc = fail(ISMC_EDEVICE, "hipMemcpyAsync", e);
+      goto out;
+    }
+  grid_argmin_kernel<<<(unsigned)M, kArgminBlock, 0, s>>>(dpR, dpT, G, dR, dT, dI);
+  if ((e = hipGetLastError()) != hipSuccess) { rc = fail(ISMC_EDEVICE, "grid_argmin_kernel", e); goto out; }
+  if ((e = hipMemcpyAsync(idx, dI, ib, hipMemcpyDeviceToHost, s)) != hipSuccess) {
     rc = fail(ISMC_EDEVICE, "hipMemcpyAsync", e);
     goto out;
   }

@@ -86,13 +86,36 @@ def make_measurements(count, packets=1_000_000, noise=0.01, seed=0, engine: Opti
             "kernel_ms": res[0].kernel_ms}
 
 
-def surrogate_seeds(sur, R_d, T_d, g=G, n=N_INDEX, t=THICK, grid=128, fine=16, polish=False):
+def grid_argmin(pR, pT, R, T, device=0):
+    """For each measurement (R[m], T[m]) the index of the grid cell with the
+    least relative misfit ((pR - R)/R)^2 + ((pT - T)/max(T, 1e-6))^2, on the
+    GPU (ismc_grid_argmin): numpy's argmin of that cost over the grid, bit
+    for bit (FP64, first index on ties)."""
+    from . import _lib
+    L = _lib.load()
+    pR = np.ascontiguousarray(pR, dtype=np.float32)
+    pT = np.ascontiguousarray(pT, dtype=np.float32)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    if pR.shape != pT.shape or R.shape != T.shape or pR.ndim != 1 or R.ndim != 1:
+        raise ValueError("pR, pT and R, T must be 1-d arrays of equal length")
+    out = np.zeros(R.size, dtype=np.int32)
+    rc = L.ismc_grid_argmin(int(device), pR.ctypes.data, pT.ctypes.data, int(pR.size), R.ctypes.data,
+                            T.ctypes.data, int(R.size), out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("ismc_grid_argmin: %s" % L.ismc_table_last_error().decode())
+    return out
+
+
+def surrogate_seeds(sur, R_d, T_d, g=G, n=N_INDEX, t=THICK, grid=128, fine=16, polish=False, argmin=None):
     """Initial (mu_a, mu_s') for each measurement from the surrogate: the
     best of a grid x grid log grid over mu_a 0.01..10 /cm, mu_s 1..1000 /cm
     (every measurement against every cell: a global search, so a seed lands
     on the branch of (R_d, T_d) nearest the measurement), then the best of a
     fine x fine grid spanning the neighbouring cells around it; optionally
-    polished by PRAXIS on the surrogate."""
+    polished by PRAXIS on the surrogate.  The global search runs on the GPU
+    (`grid_argmin`); `argmin`, a function of the same signature, replaces
+    it in host-only tests of this logic."""
     R_d = np.asarray(R_d, dtype=np.float64)
     T_d = np.asarray(T_d, dtype=np.float64)
     Tf = np.maximum(T_d, 1e-6)
@@ -111,15 +134,10 @@ def surrogate_seeds(sur, R_d, T_d, g=G, n=N_INDEX, t=THICK, grid=128, fine=16, p
     ls = np.linspace(0, 3, grid)
     A, Sg = np.meshgrid(la, ls, indexing="ij")
     pred = sur.predict(10 ** A.ravel(), 10 ** Sg.ravel(), g, n, t)
-    pR, pT = pred["R_d"], pred["T_d"]
-    ba = np.zeros(R_d.size)
-    bs = np.zeros(R_d.size)
-    for s in range(0, R_d.size, 256):
-        r = R_d[s:s + 256, None]
-        tt = T_d[s:s + 256, None]
-        c = ((pR[None, :] - r) / r) ** 2 + ((pT[None, :] - tt) / Tf[s:s + 256, None]) ** 2
-        k = np.argmin(c, axis=1)
-        ba[s:s + 256], bs[s:s + 256] = A.ravel()[k], Sg.ravel()[k]
+    # every measurement against every cell, on the GPU
+    search = argmin or (lambda pR, pT, R, T: grid_argmin(pR, pT, R, T, device=sur.mlp.device))
+    k = search(pred["R_d"], pred["T_d"], R_d, T_d)
+    ba, bs = A.ravel()[k], Sg.ravel()[k]
     if fine > 1:
         ha, hs = la[1] - la[0], ls[1] - ls[0]
         off = np.linspace(-1, 1, fine)

@@ -92,6 +92,7 @@ class MLP:
     """Handle on one ismc_mlp (weights live in device memory)."""
 
     def __init__(self, n_in=5, hidden=256, depth=4, n_out=4, max_batch=4096, device=0):
+        self.device = device
         self._L = _lib.load()
         L = self._L
         L.ismc_mlp_create.argtypes = [ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -208,7 +209,7 @@ class Surrogate:
                  ranges=RANGES, info=np.array([repr(self.train_info)]))
 
     @classmethod
-    def load(cls, path, device=0, max_batch=4096):
+    def load(cls, path, device=0, max_batch=65536):
         d = np.load(path, allow_pickle=False)
         arch = tuple(int(v) for v in d["arch"])
         mlp = MLP(*arch, max_batch=max_batch, device=device)

@@ -68,3 +68,35 @@ def test_judge_flags_misses_and_fallback_repairs_them(engine, setup, fallback):
     after = I.fit_cost([sub[i] for i in range(0, k, 2)], fb, 1_000_000, engine)
     print("fit cost of the corrupted solutions: median %.3f -> %.4f" % (np.median(before), np.median(after)))
     assert np.median(after) < 0.1 * np.median(before)
+
+
+def test_grid_argmin_equals_numpy(engine):
+    """The GPU global seed search returns numpy's argmin of the same FP64
+    cost for every measurement: random grids, exact ties (first index
+    wins), a NaN cost (first NaN wins) and a T of 0 (floored at 1e-6)."""
+    from test_inverse_host import np_grid_argmin
+    rng = np.random.default_rng(11)
+    G, M = 128 * 128, 3000
+    pR = rng.uniform(0.0, 0.6, G).astype(np.float32)
+    pT = rng.uniform(0.0, 0.6, G).astype(np.float32)
+    R = rng.uniform(0.02, 0.6, M)
+    T = rng.uniform(0.0, 0.6, M)
+    T[:5] = 0.0
+    # ties: cells 7000 and 9000 equal to cell 100 exactly, and a
+    # measurement sitting on them
+    pR[7000] = pR[9000] = pR[100]
+    pT[7000] = pT[9000] = pT[100]
+    R[10], T[10] = float(pR[100]), float(pT[100])
+    # a tie placed before the first occurrence: the lower index must win
+    pR[50], pT[50] = pR[9000], pT[9000]
+    k = I.grid_argmin(pR, pT, R, T, device=engine.device)
+    ref = np_grid_argmin(pR, pT, R, T)
+    assert np.array_equal(k, ref)
+    assert k[10] == 50
+    # a NaN cost: numpy's argmin takes the first NaN
+    pR2 = pR.copy()
+    pR2[4000] = np.nan
+    pR2[12000] = np.nan
+    assert np.array_equal(I.grid_argmin(pR2, pT, R[:64], T[:64], device=engine.device),
+                          np_grid_argmin(pR2, pT, R[:64], T[:64]))
+    assert np.all(I.grid_argmin(pR2, pT, R[:64], T[:64], device=engine.device) == 4000)

@@ -16,6 +16,16 @@ class AnalyticSurrogate:
         return {"R_d": Rd, "T_d": Td, "T_u": 0 * Rd, "R_tot": Rd}
 
 
+def np_grid_argmin(pR, pT, R, T):
+    """The global seed search in numpy (its GPU form, grid_argmin, is
+    checked against this in test_gpu_inverse.py)."""
+    R = np.asarray(R, dtype=np.float64)
+    T = np.asarray(T, dtype=np.float64)
+    c = (((np.asarray(pR, dtype=np.float32)[None, :] - R[:, None]) / R[:, None]) ** 2 +
+         ((np.asarray(pT, dtype=np.float32)[None, :] - T[:, None]) / np.maximum(T, 1e-6)[:, None]) ** 2)
+    return np.argmin(c, axis=1)
+
+
 def test_seed_refine_check_on_analytic_model():
     eng = AnalyticEngine()
     truths = np.array([(0.05, 2.0), (0.5, 20.0), (3.0, 80.0), (0.2, 150.0)])
@@ -25,7 +35,7 @@ def test_seed_refine_check_on_analytic_model():
         probs.append(inverse.problem(t["R_d"], t["T_d"]))
     R = [p.measured.R_d for p in probs]
     T = [p.measured.T_d for p in probs]
-    seeds = inverse.surrogate_seeds(AnalyticSurrogate(), R, T, grid=64, polish=True)
+    seeds = inverse.surrogate_seeds(AnalyticSurrogate(), R, T, grid=64, polish=True, argmin=np_grid_argmin)
     assert np.allclose(seeds, truths, rtol=1e-3)
     sol, nev, st = inverse.refine_batch(probs, seeds * 1.05, packets=1000, engine=eng, tol=1e-6,
                                         step=0.1, max_evals=400)
