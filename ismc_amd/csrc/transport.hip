@@ -994,6 +994,7 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   if (use_graph) {
     const std::vector<uint64_t> key = {
         sizeof(R), (uint64_t)n_points, n_chunks, (uint64_t)chunk, (uint64_t)grid,
+        (uint64_t)a.idle_lanes, (uint64_t)a.bound_lanes,
         (uint64_t)(uintptr_t)c->points.p, (uint64_t)(uintptr_t)c->ranges.p,
         (uint64_t)(uintptr_t)c->slots.p, (uint64_t)(uintptr_t)c->counter.p,
         (uint64_t)(uintptr_t)c->h_stage, pbytes, cbytes, sbytes, seed};

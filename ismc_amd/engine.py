@@ -172,7 +172,8 @@ class Engine:
         return [_result(outs[i], ms) for i in range(npts)]
 
     def simulate_batch_array(self, configs: np.ndarray, n_packets, seed: int = 1234, streams=None,
-                             fp64: bool = False, use_graph: bool = False):
+                             fp64: bool = False, use_graph: bool = False, idle_lanes: int = 0,
+                             bound_lanes: int = 0):
         """simulate_batch on configurations already laid out as a numpy array
         of the C struct (dtype CONFIG_DTYPE), for callers that patch a few
         fields of many copies of a template.  Returns (sums [P, 5],
@@ -187,7 +188,7 @@ class Engine:
         st = None
         if streams is not None:
             st = np.ascontiguousarray(np.broadcast_to(np.asarray(streams, dtype=np.uint32), (npts,)))
-        tune = _lib.Tune(0, 0, int(bool(use_graph)), 0, 0, 0)
+        tune = _lib.Tune(0, 0, int(bool(use_graph)), int(idle_lanes), int(bound_lanes), 0)
         outs = np.zeros(npts, dtype=TALLY_DTYPE)
         P64 = ctypes.POINTER(ctypes.c_uint64)
         rc = self._L.ismc_simulate_batch(

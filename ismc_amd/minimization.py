@@ -183,6 +183,7 @@ class BatchCost:
         self.seed = seed
         self.streams = list(streams) if streams is not None else list(range(len(self.problems)))
         self.use_graph = use_graph
+        self.launch = {}  # launch-shape knobs (idle_lanes, bound_lanes) for this cost's launches
         self.evaluations = 0
         self.launches = 0
         self.packets = 0
@@ -208,7 +209,7 @@ class BatchCost:
         cfg = config_array(self.problems, X[:, 0], X[:, 1], ids, self._cfg_cache)
         st = np.asarray(self.streams, dtype=np.int64)[ids] * 4
         sums, _, npk = self.engine.simulate_batch_array(cfg, self.n_packets, seed=self.seed, streams=st,
-                                                        use_graph=self.use_graph)
+                                                        use_graph=self.use_graph, **self.launch)
         self.launches += 1
         self.packets += self.n_packets * k
         mean = sums / npk[:, None].astype(np.float64)
