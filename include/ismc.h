@@ -189,6 +189,10 @@ int ismc_mlp_load_dataset(ismc_mlp* mlp, const float* x, const float* y, int64_t
 int ismc_mlp_train(ismc_mlp* mlp, const int32_t* perm, int32_t epochs, int32_t steps_per_epoch,
                    int32_t batch, float lr, float beta1, float beta2, float eps,
                    float* epoch_loss, float* total_ms);
+/* one Adam step on a host batch (upload, forward, backward, update); *loss
+ * = the batch's mean squared error before the step */
+int ismc_mlp_train_step(ismc_mlp* mlp, const float* x, const float* y, int32_t batch, float lr,
+                        float beta1, float beta2, float eps, float* loss);
 int ismc_mlp_last_ms(ismc_mlp* mlp, float* ms);
 
 /* ---------------------------------------------------------------------

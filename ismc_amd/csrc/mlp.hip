@@ -418,6 +418,24 @@ static int upload_batch(ismc_mlp* p, const float* x, const float* y, int B) {
   return 0;
 }
 
+// one Adam step on every layer from the gradients in p->g
+static int adam_all(ismc_mlp* p, float lr, float b1, float b2, float eps) {
+  p->steps++;
+  const float bc1 = 1.f - powf(b1, (float)p->steps), bc2 = 1.f - powf(b2, (float)p->steps);
+  const float lr_t = lr * sqrtf(bc2) / bc1;
+  for (const MlpLayer& Ly : p->L) {
+    const size_t nw = (size_t)Ly.K * Ly.N;
+    adam_step<<<(unsigned)((nw + 255) / 256), 256, 0, p->stream>>>(
+        p->w + Ly.w_off, p->wb + Ly.wb_off, p->g + Ly.w_off, p->m + Ly.w_off, p->v + Ly.w_off,
+        p->mask + Ly.w_off, nw, lr_t, b1, b2, eps);
+    adam_step<<<(unsigned)((Ly.N + 255) / 256), 256, 0, p->stream>>>(
+        p->w + Ly.b_off, nullptr, p->g + Ly.b_off, p->m + Ly.b_off, p->v + Ly.b_off, p->mask + Ly.b_off,
+        (size_t)Ly.N, lr_t, b1, b2, eps);
+  }
+  MCHECK(p, hipGetLastError());
+  return 0;
+}
+
 static int check_batch(ismc_mlp* p, int B) {
   if (B <= 0 || B > p->max_batch || B % TM)
     return mlp_err(p, ISMC_EINVAL, "batch must be a positive multiple of 64 and <= max_batch");
@@ -658,19 +676,7 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
                                               p->in_pad, p->X0, p->Tb);
       MCHECK(p, hipGetLastError());
       if ((rc = forward(p, B)) || (rc = backward(p, B, nullptr, lossd + (size_t)s * nb))) return rc;
-      p->steps++;
-      const float bc1 = 1.f - powf(b1, (float)p->steps), bc2 = 1.f - powf(b2, (float)p->steps);
-      const float lr_t = lr * sqrtf(bc2) / bc1;
-      for (const MlpLayer& Ly : p->L) {
-        const size_t nw = (size_t)Ly.K * Ly.N;
-        adam_step<<<(unsigned)((nw + 255) / 256), 256, 0, p->stream>>>(
-            p->w + Ly.w_off, p->wb + Ly.wb_off, p->g + Ly.w_off, p->m + Ly.w_off, p->v + Ly.w_off,
-            p->mask + Ly.w_off, nw, lr_t, b1, b2, eps);
-        adam_step<<<(unsigned)((Ly.N + 255) / 256), 256, 0, p->stream>>>(
-            p->w + Ly.b_off, nullptr, p->g + Ly.b_off, p->m + Ly.b_off, p->v + Ly.b_off, p->mask + Ly.b_off,
-            (size_t)Ly.N, lr_t, b1, b2, eps);
-      }
-      MCHECK(p, hipGetLastError());
+      if ((rc = adam_all(p, lr, b1, b2, eps))) return rc;
     }
     MCHECK(p, hipEventRecord(p->e1, p->stream));
     MCHECK(p, hipMemcpyAsync(lp.data(), lossd, sizeof(float) * lp.size(), hipMemcpyDeviceToHost, p->stream));
@@ -682,6 +688,22 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
     if (epoch_loss) epoch_loss[e] = (float)(esum / steps_per_epoch);
   }
   if (total_ms) *total_ms = ms_total;
+  return 0;
+}
+
+// One Adam step on a host batch x[B][n_in], y[B][n_out] (targets in the
+// output's units): upload, forward, backward, update; *loss (if non-NULL)
+// is the batch's mean squared error before the step.
+int ismc_mlp_train_step(ismc_mlp* p, const float* x, const float* y, int32_t B, float lr, float b1,
+                        float b2, float eps, float* loss) {
+  if (!p || !x || !y) return ISMC_EINVAL;
+  int rc;
+  if ((rc = check_batch(p, B))) return rc;
+  MCHECK(p, hipSetDevice(p->device));
+  if ((rc = upload_batch(p, x, y, B))) return rc;
+  if ((rc = forward(p, B)) || (rc = backward(p, B, loss))) return rc;
+  if ((rc = adam_all(p, lr, b1, b2, eps))) return rc;
+  MCHECK(p, hipStreamSynchronize(p->stream));
   return 0;
 }
 

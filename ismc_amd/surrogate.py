@@ -118,6 +118,10 @@ class MLP:
                                      ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
         L.ismc_mlp_train.restype = ctypes.c_int
+        L.ismc_mlp_train_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                          ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                          ctypes.POINTER(ctypes.c_float)]
+        L.ismc_mlp_train_step.restype = ctypes.c_int
         L.ismc_mlp_last_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
         L.ismc_mlp_last_ms.restype = ctypes.c_int
         self._h = ctypes.c_void_p()
@@ -173,6 +177,16 @@ class MLP:
         self._check(self._L.ismc_mlp_gradients(self._h, x.ctypes.data, y.ctypes.data, x.shape[0],
                                                g.ctypes.data, ctypes.byref(loss)), "gradients")
         return float(loss.value), g
+
+    def train_step(self, x, y, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        """One Adam step on a host batch (rows a multiple of 64); returns the
+        batch's mean squared error before the step."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(self._L.ismc_mlp_train_step(self._h, x.ctypes.data, y.ctypes.data, x.shape[0], lr, beta1,
+                                                beta2, eps, ctypes.byref(loss)), "train_step")
+        return float(loss.value)
 
     def load_dataset(self, x, y):
         self._x = np.ascontiguousarray(x, dtype=np.float32)

@@ -60,6 +60,34 @@ def test_gradients_match_fp64_oracle(mlp):
             o += size
 
 
+def test_train_step_is_one_adam_step_on_the_gradients():
+    """ismc_mlp_train_step on a host batch equals ismc_mlp_gradients on the
+    same batch followed by Adam's first step written out in numpy (zero
+    moments: m = (1-b1) g, v = (1-b2) g^2, step size lr sqrt(1-b2)/(1-b1));
+    its loss is the batch's loss before the step."""
+    m = S.MLP(*ARCH, max_batch=4096)
+    try:
+        w0 = S.init_params(ARCH, seed=9)
+        m.set_params(w0)
+        rng = np.random.default_rng(4)
+        x = rng.uniform(-1, 1, (1024, 5)).astype(np.float32)
+        t = rng.normal(0, 1, (1024, 4)).astype(np.float32)
+        loss_g, g = m.gradients(x, t)
+        lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+        loss = m.train_step(x, t, lr=lr, beta1=b1, beta2=b2, eps=eps)
+        w1 = m.get_params()
+        f = np.float32
+        mm = f(1 - b1) * g
+        vv = f(1 - b2) * g * g
+        lr_t = f(lr * np.sqrt(1 - b2) / (1 - b1))
+        expect = w0 - lr_t * mm / (np.sqrt(vv) + f(eps))
+        assert loss == pytest.approx(loss_g, rel=1e-6)
+        assert np.allclose(w1, expect, rtol=1e-5, atol=1e-7)
+        assert np.abs(w1 - w0).max() > 0.5 * lr
+    finally:
+        m.close()
+
+
 def test_training_fits_a_smooth_function(mlp):
     rng = np.random.default_rng(2)
     x = rng.uniform(-1, 1, (65536, 5)).astype(np.float32)
