@@ -228,7 +228,7 @@ def main():
                     help="also run a plain HBM copy of this many bytes (byte-counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X "
+        print("bench.py measures one GPU only: the transport runs on a single MI355X here "
               "(packets shard by packet id across GPUs, with one sum of the tallies at the end)", file=sys.stderr)
         sys.exit(2)
 

@@ -277,7 +277,7 @@ def refine_newton(problems, seeds, sur, iters=6, packets=100_000, seed=7, engine
 def refine_mc_newton(problems, start, iters=8, packets=100_000, seed=7, engine: Optional[Engine] = None,
                      stream_base=0, h=0.05, radius=0.5, s_cut=1e-3, step_tol=1e-3, log=None):
     """Gauss-Newton on the MC residuals alone, for solves the surrogate
-    cannot guide (the fallback after the judge).  y = (ln mu_a, ln mu_s'),
+    cannot guide (the fallback after the MC check).  y = (ln mu_a, ln mu_s'),
     r = (MC(y) - M) / M the relative residuals of R_d and T_d.  Each
     iteration is one launch that runs, for every pending solve, its trial
     point y and the points y + h e_1, y + h e_2 on the solve's own Philox
