@@ -215,3 +215,47 @@ def test_errors_are_raised_not_exited(engine):
         engine.simulate(slab_between_slides(), (1 << 32) + 1)
     ok = engine.simulate(slab_between_slides(), 1000)
     assert ok.n_packets == 1000
+
+
+def test_reference_style_ctypes_binding(engine):
+    """The ctypes stub a maintainer adds for the reference's host function
+    ISMLMC(dev, N, layers, n, g, t, bounds, mu_a, mu_s, mu_t, R, ...)
+    (CUDA/CUDAISMLMC.h:656), written as the integration notes give it,
+    returns the Engine's tallies on the same seed."""
+    import ctypes
+    from ismc_amd._lib import Config, Tally, load
+
+    lib = load()
+    ctx = ctypes.c_void_p()
+    assert lib.ismc_create(0, ctypes.byref(ctx)) == 0
+
+    def ISMLMC_ref(dev, N, layers, n, g, t, bounds, mu_a, mu_s, mu_t, R, R_pw, R_fs, R_fp, R_fd,
+                   R_f, R_angle, specular_included, T, T_pw, T_fs, T_fp, T_fd, T_f, T_angle, seed=1234):
+        c = Config()
+        c.n_layers, c.specular_included = layers, int(specular_included)
+        c.n_ambient, c.roulette_threshold, c.roulette_m = 1.0, 1e-4, 10.0
+        for i in range(layers):
+            c.n[i], c.g[i], c.t[i] = n[i], g[i], t[i]
+            c.mu_a[i], c.mu_s[i], c.mu_t[i] = mu_a[i], mu_s[i], mu_t[i]
+        for S, v in ((c.R, (R, R_pw, R_fs, R_fp, R_fd, R_f, R_angle)),
+                     (c.T, (T, T_pw, T_fs, T_fp, T_fd, T_f, T_angle))):
+            S.present, S.pw, S.fs, S.fp, S.fd, S.f, S.angle = int(v[0]), *v[1:]
+        out = Tally()
+        rc = lib.ismc_simulate(ctx, ctypes.byref(c), N, seed, 0, 0, 0, ctypes.byref(out))
+        if rc != 0:
+            raise RuntimeError(lib.ismc_last_error(ctx).decode())
+        return tuple(out.sum[k] / N for k in range(5))   # (A, R_d, R_s, T_d, T_u)
+
+    try:
+        cfg = slab_between_slides()
+        L = cfg.layers
+        bounds = list(np.cumsum([0.0] + [x.t for x in L]))
+        sp = lambda S: (S.present, S.pw, S.fs, S.fp, S.fd, S.f_total, S.angle)
+        N = 200_000
+        got = ISMLMC_ref(0, N, len(L), [x.n for x in L], [x.g for x in L], [x.t for x in L], bounds,
+                         [x.mu_a for x in L], [x.mu_s for x in L], [x.mu_a + x.mu_s for x in L],
+                         *sp(cfg.R), cfg.specular_included, *sp(cfg.T), seed=99)
+        ref = engine.simulate(cfg, N, seed=99)
+        assert got == tuple(ref.mean[k] for k in ("A", "R_d", "R_s", "T_d", "T_u"))
+    finally:
+        lib.ismc_destroy(ctx)
