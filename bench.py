@@ -288,6 +288,9 @@ def main():
                      "frac": achieved / peak, "traffic": traffic,
                      "traffic_note": "HBM bytes per launch: rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00 "
                                      "(copy-calibrated) at 2^28 packets, per packet x N",
+                     "peak_note": "n_CU x 4 SIMDs x 32 lanes x clock: a SIMD issues a full-rate FP32 "
+                                  "wave64 instruction in 2 cycles (ismc_issue_rate); at 64 lanes per CU "
+                                  "per cycle this fraction would be %.2f" % (2 * achieved / peak),
                      "slots_per_packet": W, "kernel_ms_avg": avg_s * 1e3,
                      "kernel_packets_per_s": N / avg_s,
                      "events_per_packet": {k: v / N for k, v in events.items()}},
