@@ -193,8 +193,10 @@ def inverse_bench(eng, count):
                                                 plain_tol=1e-3))):
         r = inverse.run(sur, engine=eng, methods=methods, with_ablation=False, log=log, **kw)
         p = r["plain"]
+        plain_ok = p["solves_per_s"] * p["errors"]["frac_both_within_1pct"]
         entry = {"measurements": r["count"], "noise": r["noise"],
                  "plain_solves_per_s": p["solves_per_s"], "plain_solves": p["count"],
+                 "plain_within_1pct_solves_per_s": plain_ok,
                  "plain_fits_as_well_as_truth": p["fits_as_well_as_truth"], "plain_errors": p["errors"],
                  "packets": {"measurement": r["meas_packets"], "refinement": r["refine_packets"],
                              "judge": r["check_packets"], "plain_per_evaluation": r["plain_packets"]}}
@@ -202,6 +204,11 @@ def inverse_bench(eng, count):
             n = r["seeded_" + m]
             entry[{"newton": "defect_correction", "praxis": "praxis"}[m]] = {
                 "solves_per_s": n["solves_per_s"], "speedup_vs_plain": r["speedup_%s_vs_plain" % m],
+                # solves that recover both mu_a and mu_s' within 1 %, per second, and that
+                # rate against the plain solver's
+                "within_1pct_solves_per_s": n["solves_per_s"] * n["errors"]["frac_both_within_1pct"],
+                "within_1pct_speedup_vs_plain": (n["solves_per_s"] * n["errors"]["frac_both_within_1pct"]
+                                                 / plain_ok) if plain_ok > 0 else None,
                 "fits_as_well_as_truth": n["fits_as_well_as_truth"], "errors": n["errors"],
                 "misses": n["misses"], "misses_repaired_by_fallback": n["misses_repaired_by_fallback"],
                 "fallback": n["fallback"], "mc_evaluations_per_solve": n["mean_evals"],
