@@ -259,3 +259,29 @@ def test_reference_style_ctypes_binding(engine):
         assert got == tuple(ref.mean[k] for k in ("A", "R_d", "R_s", "T_d", "T_u"))
     finally:
         lib.ismc_destroy(ctx)
+
+
+def test_error_returns_and_zero_packets(engine):
+    """Host-side failures come back as status codes with a message, before
+    any launch (the reference exits the process): more than 2^32 packets in
+    one point and a packet-id range that wraps are ERANGE, an invalid
+    configuration or a null array EINVAL; a call for 0 packets returns
+    empty tallies.  The context stays usable after each."""
+    import ctypes
+    from ismc_amd import _lib
+    cfg = single_layer(1.0, 10.0)
+    with pytest.raises(ValueError, match="2\\^32"):
+        engine.simulate(cfg, (1 << 32) + 1)
+    with pytest.raises(ValueError, match="overflow"):
+        engine.simulate(cfg, 1000, first_packet_id=(1 << 64) - 10)
+    bad = SampleConfig(layers=[Layer(n=1.4, g=1.0, t=0.1, mu_a=1.0, mu_s=10.0)], R=cfg.R, T=cfg.T)
+    with pytest.raises(ValueError):
+        engine.simulate(bad, 1000)
+    L = _lib.load()
+    out = (_lib.Tally * 1)()
+    rc = L.ismc_simulate_batch(engine._ctx, None, None, None, None, 1, 1, 0, None, out)
+    assert rc == _lib.EINVAL and L.ismc_last_error(engine._ctx)
+    z = engine.simulate(cfg, 0)
+    assert z.n_packets == 0 and all(v == 0 for v in z.sum.values())
+    ok = engine.simulate(cfg, 5000, seed=3)
+    assert ok.n_packets == 5000 and 0.99 < sum(ok.mean.values()) < 1.01
