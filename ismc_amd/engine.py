@@ -173,7 +173,8 @@ class Engine:
 
     def simulate_batch_array(self, configs: np.ndarray, n_packets, seed: int = 1234, streams=None,
                              fp64: bool = False, use_graph: bool = False, idle_lanes: int = 0,
-                             bound_lanes: int = 0):
+                             bound_lanes: int = 0, first_packet_ids=None, chunk_packets: int = 0,
+                             blocks_per_cu: int = 0):
         """simulate_batch on configurations already laid out as a numpy array
         of the C struct (dtype CONFIG_DTYPE), for callers that patch a few
         fields of many copies of a template.  Returns (sums [P, 5],
@@ -188,11 +189,16 @@ class Engine:
         st = None
         if streams is not None:
             st = np.ascontiguousarray(np.broadcast_to(np.asarray(streams, dtype=np.uint32), (npts,)))
-        tune = _lib.Tune(0, 0, int(bool(use_graph)), int(idle_lanes), int(bound_lanes), 0)
+        first = None
+        if first_packet_ids is not None:
+            first = np.ascontiguousarray(np.broadcast_to(np.asarray(first_packet_ids, dtype=np.uint64), (npts,)))
+        tune = _lib.Tune(int(blocks_per_cu), int(chunk_packets), int(bool(use_graph)), int(idle_lanes),
+                         int(bound_lanes), 0)
         outs = np.zeros(npts, dtype=TALLY_DTYPE)
         P64 = ctypes.POINTER(ctypes.c_uint64)
         rc = self._L.ismc_simulate_batch(
-            self._ctx, configs.ctypes.data_as(ctypes.POINTER(_lib.Config)), npk.ctypes.data_as(P64), None,
+            self._ctx, configs.ctypes.data_as(ctypes.POINTER(_lib.Config)), npk.ctypes.data_as(P64),
+            first.ctypes.data_as(P64) if first is not None else None,
             st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if st is not None else None, npts, int(seed),
             self._flags(fp64, False), ctypes.byref(tune), outs.ctypes.data_as(ctypes.POINTER(_lib.Tally)))
         _lib.check(self._ctx, rc, "ismc_simulate_batch")

@@ -21,7 +21,7 @@ from typing import List, Optional
 import numpy as np
 
 from .config import Layer, SampleConfig, Sphere, mus_from_reduced
-from .engine import Engine, default_engine
+from .engine import CONFIG_DTYPE, TALLIES, Engine, default_engine
 
 
 @dataclass
@@ -218,36 +218,46 @@ def adaptive_sweep(cfgs: List[SampleConfig], rse=0.005, pilot=10_000, n_min=10_0
     (a packet leaves its weight in exactly one exit tally, so
     Var(R_d + R_s) = E[R_d^2] + E[R_s^2] - R^2); each point then gets
     N = clamp(ceil(Var / (R rse)^2), n_min, n_max) packets in total, the
-    remaining N - pilot in one launch on packet ids after the pilot's."""
+    remaining N - pilot in one launch on packet ids after the pilot's.
+    The configurations go to the library once, as an array, and the tallies
+    come back as arrays (no per-point objects until the result list)."""
     eng = engine or default_engine()
     npts = len(cfgs)
     streams = np.arange(npts, dtype=np.uint32)
-    p = eng.simulate_batch(cfgs, pilot, seed=seed, streams=streams, chunk_packets=chunk_packets,
-                           blocks_per_cu=blocks_per_cu, autotune=autotune)
-    need = np.zeros(npts, dtype=np.int64)
-    for k, r in enumerate(p):
-        R = r.mean["R_d"] + r.mean["R_s"]
-        var = (r.sumsq["R_d"] + r.sumsq["R_s"]) / r.n_packets - R * R
-        n = n_max if R <= 0 else int(math.ceil(max(var, 0.0) / (R * rse) ** 2))
-        need[k] = min(max(n, n_min), n_max)
+    arr = np.frombuffer(b"".join(bytes(c.to_c()) for c in cfgs), dtype=CONFIG_DTYPE).copy()
+
+    def launch(n_packets, first=None):
+        kw = {"chunk_packets": chunk_packets, "blocks_per_cu": blocks_per_cu}
+        if autotune:
+            kw = eng.tuner.params(cfgs, n_packets, seed=seed)
+        out = eng.simulate_batch_array(arr, n_packets, seed=seed, streams=streams, first_packet_ids=first, **kw)
+        return out, eng.last_launch()["kernel_ms"]
+
+    iRd, iRs = TALLIES.index("R_d"), TALLIES.index("R_s")
+    (s1, q1, n1), pilot_ms = launch(pilot)
+    n1 = n1.astype(np.float64)
+    # the plan, in the scalar form's order of operations (same packet counts)
+    R = s1[:, iRd] / n1 + s1[:, iRs] / n1
+    var = (q1[:, iRd] + q1[:, iRs]) / n1 - R * R
+    with np.errstate(divide="ignore", invalid="ignore"):
+        n = np.where(R <= 0, float(n_max), np.ceil(np.maximum(var, 0.0) / (R * rse) ** 2))
+    need = np.clip(n, n_min, n_max).astype(np.int64)
     extra = np.maximum(need - pilot, 0)
-    m = eng.simulate_batch(cfgs, extra.astype(np.uint64), seed=seed, streams=streams,
-                           first_packet_ids=np.full(npts, pilot, dtype=np.uint64),
-                           chunk_packets=chunk_packets, blocks_per_cu=blocks_per_cu,
-                           autotune=autotune) \
-        if extra.sum() > 0 else None
-    out = []
-    for k in range(npts):
-        a = p[k]
-        tot_n = a.n_packets + (m[k].n_packets if m is not None else 0)
-        s = {t: a.sum[t] + (m[k].sum[t] if m is not None else 0.0) for t in a.sum}
-        q = {t: a.sumsq[t] + (m[k].sumsq[t] if m is not None else 0.0) for t in a.sum}
-        mean = {t: s[t] / tot_n for t in s}
-        se = {t: math.sqrt(max(q[t] / tot_n - mean[t] ** 2, 0.0) / max(tot_n - 1, 1)) for t in s}
-        R = mean["R_d"] + mean["R_s"]
-        varR = (q["R_d"] + q["R_s"]) / tot_n - R * R
-        out.append({"n_packets": tot_n, "mean": mean, "se": se,
-                    "rse_R": math.sqrt(max(varR, 0) / tot_n) / R if R > 0 else float("inf")})
-    info = {"pilot_kernel_ms": p[0].kernel_ms, "main_kernel_ms": m[0].kernel_ms if m is not None else 0.0,
+    if extra.sum() > 0:
+        (s2, q2, n2), main_ms = launch(extra.astype(np.uint64), first=np.full(npts, pilot, dtype=np.uint64))
+    else:
+        s2, q2, n2, main_ms = np.zeros_like(s1), np.zeros_like(q1), np.zeros(npts, dtype=np.uint64), 0.0
+    tot = (n1 + n2.astype(np.float64))[:, None]
+    S, Q = s1 + s2, q1 + q2
+    mean = S / tot
+    se = np.sqrt(np.maximum(Q / tot - mean ** 2, 0.0) / np.maximum(tot - 1, 1))
+    Rm = mean[:, iRd] + mean[:, iRs]
+    varR = (Q[:, iRd] + Q[:, iRs]) / tot[:, 0] - Rm * Rm
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rse_R = np.where(Rm > 0, np.sqrt(np.maximum(varR, 0) / tot[:, 0]) / Rm, float("inf"))
+    mean_l, se_l, tot_l, rse_l = mean.tolist(), se.tolist(), tot[:, 0].astype(np.int64).tolist(), rse_R.tolist()
+    out = [{"n_packets": tot_l[k], "mean": dict(zip(TALLIES, mean_l[k])), "se": dict(zip(TALLIES, se_l[k])),
+            "rse_R": rse_l[k]} for k in range(npts)]
+    info = {"pilot_kernel_ms": pilot_ms, "main_kernel_ms": main_ms,
             "total_packets": int(pilot * npts + extra.sum()), "planned": need}
     return out, info

@@ -76,6 +76,38 @@ def test_adaptive_sweep_reaches_target_rse(engine):
     assert info["total_packets"] == sum(o["n_packets"] for o in out)
 
 
+def test_adaptive_sweep_matches_per_point_plan(engine):
+    """The array form of the sweep plans and reports exactly what the
+    per-point form does: a pilot launch, N = clamp(ceil(Var / (R rse)^2)),
+    the rest on packet ids after the pilot's; and the autotuned launch
+    shape changes nothing."""
+    import math
+    mua, mus, cfgs = lookup_grid(3, 4)
+    pilot, n_min, n_max, rse = 5000, 8000, 400_000, 0.004
+    out, info = lookup.adaptive_sweep(cfgs, rse=rse, pilot=pilot, n_min=n_min, n_max=n_max, seed=21, engine=engine)
+    tuned, _ = lookup.adaptive_sweep(cfgs, rse=rse, pilot=pilot, n_min=n_min, n_max=n_max, seed=21, engine=engine,
+                                     autotune=True)
+    streams = np.arange(len(cfgs), dtype=np.uint32)
+    p = engine.simulate_batch(cfgs, pilot, seed=21, streams=streams)
+    need = []
+    for r in p:
+        R = r.mean["R_d"] + r.mean["R_s"]
+        var = (r.sumsq["R_d"] + r.sumsq["R_s"]) / r.n_packets - R * R
+        n = n_max if R <= 0 else int(math.ceil(max(var, 0.0) / (R * rse) ** 2))
+        need.append(min(max(n, n_min), n_max))
+    assert list(info["planned"]) == need
+    extra = np.maximum(np.array(need) - pilot, 0).astype(np.uint64)
+    m = engine.simulate_batch(cfgs, extra, seed=21, streams=streams,
+                              first_packet_ids=np.full(len(cfgs), pilot, dtype=np.uint64))
+    for k, o in enumerate(out):
+        tot = p[k].n_packets + m[k].n_packets
+        assert o["n_packets"] == tot == need[k]
+        for t in o["mean"]:
+            assert o["mean"][t] == (p[k].sum[t] + m[k].sum[t]) / tot
+        assert tuned[k] == o
+    assert info["total_packets"] == sum(need)
+
+
 def test_run_lookup_from_ini(engine, tmp_path):
     f = ini_mod.load(os.path.join(HERE, "golden", "sample_lookup.ini"))
     out = lookup.run_lookup(f, seed=2, engine=engine, rng_seed=4)
