@@ -59,3 +59,61 @@ def test_csv_layout(tmp_path):
     assert lines[4].startswith('"Standard Error in Diffuse Reflectance"')
     m, s = lookup.read_csv_table(str(p), 2)
     assert np.allclose(m, d, rtol=1e-6) and np.allclose(s, d / 10, rtol=1e-6)
+
+
+class BernoulliEngine:
+    """Stand-in for the batch launch: every packet leaves weight 1 in R_d
+    with probability p_R and in R_s with probability 0.02 (so sum == sum of
+    squares), p_R depending on the point's mu_a; records each launch."""
+
+    def __init__(self, p_R):
+        self.p_R = np.asarray(p_R, dtype=np.float64)
+        self.launches = []
+
+    def simulate_batch_array(self, arr, n_packets, seed=0, streams=None, first_packet_ids=None, **kw):
+        npts = arr.shape[0]
+        n = np.broadcast_to(np.asarray(n_packets, dtype=np.uint64), (npts,)).copy()
+        first = None if first_packet_ids is None else np.asarray(first_packet_ids).copy()
+        self.launches.append({"n": n, "first": first, "kw": kw, "mu_a": arr["mu_a"][:, 0].copy()})
+        s = np.zeros((npts, 5))
+        s[:, 1] = self.p_R * n
+        s[:, 2] = 0.02 * n * (self.p_R > 0)
+        return s, s.copy(), n
+
+    def last_launch(self):
+        return {"kernel_ms": 1.5}
+
+
+def test_adaptive_sweep_plan_and_statistics():
+    """The pilot estimates R = R_d + R_s and its per-packet variance; each
+    point gets N = clamp(ceil(Var / (R rse)^2), n_min, n_max) packets, the
+    rest of them in a second launch on packet ids after the pilot's."""
+    from ismc_amd.workloads import lookup_grid
+    _, _, cfgs = lookup_grid(2, 3)
+    p_R = np.array([0.3, 0.0, 0.05, 0.9, 0.6, 0.001])
+    eng = BernoulliEngine(p_R)
+    pilot, n_min, n_max, rse = 1000, 2000, 500_000, 0.01
+    out, info = lookup.adaptive_sweep(cfgs, rse=rse, pilot=pilot, n_min=n_min, n_max=n_max, seed=3,
+                                      engine=eng, chunk_packets=512)
+    need = []
+    for p in p_R:
+        R = p + 0.02 * (p > 0)
+        var = R - R * R                                   # weights 0/1 in one tally or none
+        n = n_max if R <= 0 else math.ceil(max(var, 0.0) / (R * rse) ** 2)
+        need.append(min(max(n, n_min), n_max))
+    assert list(info["planned"]) == need
+    assert len(eng.launches) == 2
+    assert np.all(eng.launches[0]["n"] == pilot) and eng.launches[0]["first"] is None
+    assert list(eng.launches[1]["n"]) == [max(k - pilot, 0) for k in need]
+    assert np.all(eng.launches[1]["first"] == pilot)
+    assert eng.launches[0]["kw"] == {"chunk_packets": 512, "blocks_per_cu": 0}
+    assert info["total_packets"] == sum(need) and info["pilot_kernel_ms"] == 1.5
+    for k, o in enumerate(out):
+        R = p_R[k] + 0.02 * (p_R[k] > 0)
+        assert o["n_packets"] == need[k]
+        assert o["mean"]["R_d"] == pytest.approx(p_R[k], rel=1e-12)
+        if R > 0:
+            assert o["rse_R"] == pytest.approx(math.sqrt((R - R * R) / need[k]) / R, rel=1e-9)
+            assert o["rse_R"] <= rse * 1.000001 or need[k] == n_max
+        else:
+            assert o["rse_R"] == float("inf") and need[k] == n_max
