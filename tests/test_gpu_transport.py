@@ -154,6 +154,26 @@ def test_large_run_properties(engine):
     assert ev["entrances"] == ev["sphere_decisions"] and ev["reinjections"] < ev["entrances"]
 
 
+def test_full_size_step_is_additive_over_packet_ranges(engine):
+    # the bench step's 2^30 packets equal its two halves run as separate
+    # points on disjoint packet-id ranges of the same stream: event counts
+    # exactly, tallies to the rounding of the FP64 conversion of the 64-bit
+    # fixed-point sums; energy balances and no packet is dropped
+    cfg = slab_between_slides()
+    N = 1 << 30
+    full = engine.simulate(cfg, N, seed=1234)
+    h = engine.simulate_batch([cfg, cfg], [N // 2, N // 2], seed=1234, first_packet_ids=[0, N // 2],
+                              streams=[0, 0])
+    assert full.n_aborted == 0 and h[0].n_aborted == 0 and h[1].n_aborted == 0
+    for k in full.events:
+        assert full.events[k] == h[0].events[k] + h[1].events[k]
+    for k in full.sum:
+        assert full.sum[k] == pytest.approx(h[0].sum[k] + h[1].sum[k], rel=1e-15, abs=1e-9)
+        assert full.sumsq[k] == pytest.approx(h[0].sumsq[k] + h[1].sumsq[k], rel=1e-15, abs=1e-9)
+    assert abs(sum(full.mean.values()) - 1.0) < 1e-4
+    assert h[0].sum != h[1].sum  # the halves are different packets
+
+
 @pytest.mark.parametrize("name,cfg", [("slab_between_slides", slab_between_slides()),
                                       ("three_layer", three_layer_sample())])
 def test_fp32_vs_fp64_kernel_deviation(engine, name, cfg):
