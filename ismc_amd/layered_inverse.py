@@ -155,8 +155,8 @@ def solve(problems: Sequence[LayeredProblem], guesses, packets=1_000_000, step=0
     """PRAXIS per problem in log coordinates from `guesses` (linear units),
     all problems in lock step.  With `restarts`, PRAXIS starts again from
     its own result (fresh directions, step `step`, the same cost surface)
-    that many times: in four or more dimensions a long narrow valley can
-    stop it early.  Returns (solutions [P, n] linear, costs, evaluations
+    that many times: in three or more dimensions a long narrow valley of
+    the MC cost can stop it early.  Returns (solutions [P, n] linear, costs, evaluations
     per solve, stats)."""
     eng = engine or default_engine()
     if len({len(p.unknowns) for p in problems}) != 1:

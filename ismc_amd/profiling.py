@@ -12,8 +12,8 @@ HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (each scaled by what the
 same counter reads for the plain copy kernel of known size), and the
 activity counters per packet.
 
-    python -m ismc_amd.profiling --packets 268435456 --copy-bytes 1073741824 \\
-        --trace KT --fetch F --write W --pmc A --pmc B > counters.json
+    python -m ismc_amd.profiling --packets 268435456 --trace-packets 1073741824 \\
+        --copy-bytes 1073741824 --trace KT --fetch F --write W --pmc A --pmc B > counters.json
 
 `--issue-rates` instead measures the device's VALU issue cost per wave64
 instruction for single instruction kinds (ismc_issue_rate) and prints them
@@ -66,15 +66,21 @@ def kernel_stats(directory):
             for r in _csv(directory, "kernel_stats.csv")]
 
 
-def summarize(packets, copy_bytes, trace=None, fetch=None, write=None, pmc=(), kernel=KERNEL):
+def summarize(packets, copy_bytes, trace=None, fetch=None, write=None, pmc=(), kernel=KERNEL,
+              trace_packets=None):
+    """`packets`: packets per launch of the counter runs (the per-packet
+    figures); `trace_packets`: per launch of the kernel-trace run, if that
+    run used another size (the packets/s figure)."""
     out = {"kernel": kernel, "packets": packets}
     if trace:
         st = kernel_stats(trace)
         out["trace"] = st
         k = [s for s in st if kernel in s["name"]]
         if k:
+            tp = trace_packets or packets
             out["avg_ns"] = k[0]["avg_ns"]
-            out["packets_per_s"] = packets / (k[0]["avg_ns"] * 1e-9)
+            out["trace_packets"] = tp
+            out["packets_per_s"] = tp / (k[0]["avg_ns"] * 1e-9)
     if fetch and write:
         f_copy = counters(fetch, COPY)["FETCH_SIZE"]
         w_copy = counters(write, COPY)["WRITE_SIZE"]
@@ -135,7 +141,8 @@ def main(argv=None):
     ap.add_argument("--issue-rates", action="store_true",
                     help="measure VALU issue costs per instruction kind instead")
     ap.add_argument("--device", type=int, default=0)
-    ap.add_argument("--packets", type=int, help="packets per profiled launch")
+    ap.add_argument("--packets", type=int, help="packets per launch of the counter runs")
+    ap.add_argument("--trace-packets", type=int, help="packets per launch of the trace run (default --packets)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--trace")
     ap.add_argument("--fetch")
@@ -149,8 +156,8 @@ def main(argv=None):
         return
     if a.packets is None:
         ap.error("--packets is required")
-    json.dump(summarize(a.packets, a.copy_bytes, a.trace, a.fetch, a.write, a.pmc, a.kernel),
-              sys.stdout, indent=1)
+    json.dump(summarize(a.packets, a.copy_bytes, a.trace, a.fetch, a.write, a.pmc, a.kernel,
+                        a.trace_packets), sys.stdout, indent=1)
     print()
 
 

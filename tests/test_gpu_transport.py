@@ -63,7 +63,7 @@ def test_eval_physics_matches_oracle(engine):
     refh = np.array([O.hg_cos(g, u) for g, u, _, _ in hg])
     assert np.max(np.abs(engine.eval_physics(1, hg, fp64=True)[:, 0] - refh)) < 1e-12
     assert np.max(np.abs(engine.eval_physics(1, hg)[:, 0] - refh)) < 5e-5
-    # cos(2 pi u) and -ln(u) at 24-bit grid points
+    # cos(2 pi u) and -ln(u) at points of the uniforms' 2^-23 grid
     u = (np.arange(1, 4097) * 4096.0) / 2 ** 24
     uu = np.stack([u, 0 * u, 0 * u, 0 * u], 1)
     assert np.max(np.abs(engine.eval_physics(2, uu)[:, 0] - np.cos(2 * np.pi * u))) < 2e-6
