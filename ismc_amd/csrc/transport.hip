@@ -354,6 +354,22 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     // wait for one.  So the wave does not pay for the code of every path on
     // every iteration, and every packet still draws the same Philox block
     // for the same event.
+    //
+    // A packet in flight after more than kMaxEventsPerPacket events is
+    // dropped (counted, not tallied) before it runs another.  An event
+    // index grows by at most FlightSteps per iteration and the passes run
+    // no event of their own, so only a wave with a lane within FlightSteps
+    // of the limit checks, here, and runs a single flight event in this
+    // iteration; every other wave does no per-event check.
+    const bool near_limit =
+        __ballot(ev + (uint32_t)FlightSteps<R>::value > kMaxEventsPerPacket) != 0;
+    if (near_limit) {
+      asm volatile("");  // keeps this rare block a branch, not selects run by every wave
+      if (phase == PH_FLIGHT && ev > kMaxEventsPerPacket) {
+        phase = PH_END;
+        aborted = true;
+      }
+    }
     const uint64_t fly = __ballot(phase == PH_FLIGHT);
     const uint64_t bnd = __ballot(phase == PH_BOUND);
     const uint64_t idle = __ballot(phase != PH_FLIGHT && phase != PH_BOUND && phase != PH_DONE);
@@ -534,6 +550,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     // sphere or its end stops for this iteration
 #pragma unroll
     for (int step = 0; step < FlightSteps<R>::value; ++step) {
+    if (step > 0 && near_limit) break;
     const bool run_flight = (phase == PH_FLIGHT);
     const bool run_enter = step == 0 && rare && (phase == PH_ENTER);
     if (__ballot(run_flight || run_enter) == 0) break;
@@ -605,17 +622,10 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         phase = PH_FLIGHT;
       }
     }
-    if (run_flight || run_enter) {
-      ev += 1;
-      // a packet still running after kMaxEventsPerPacket events is dropped
-      // (counted, not tallied); one waiting at an interface or for its
-      // sphere decision is checked after it, once it is known whether it
-      // continues
-      if (phase == PH_FLIGHT && ev > kMaxEventsPerPacket) {
-        phase = PH_END;
-        aborted = true;
-      }
-    }
+    // a packet that goes on flying past the event limit is dropped at the
+    // top of the next iteration; one waiting at an interface or for its
+    // sphere decision is checked when it is known whether it continues
+    if (run_flight || run_enter) ev += 1;
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

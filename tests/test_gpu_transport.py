@@ -174,6 +174,34 @@ def test_full_size_step_is_additive_over_packet_ranges(engine):
     assert h[0].sum != h[1].sum  # the halves are different packets
 
 
+@pytest.mark.parametrize("fp64", [False, True], ids=["fp32", "fp64"])
+def test_event_limit_drops_runaway_packets(engine, fp64):
+    # A matched, non-absorbing slab 20,000 mean free paths thick: about one
+    # packet in a thousand diffuses for more than 2^22 events.  Such a packet
+    # is dropped (counted, not tallied) right after its 2^22 + 1st event,
+    # whatever the launch shape; every other packet leaves its whole weight
+    # in one exit tally.  The oracle drops about as many (its long histories
+    # are not the kernel's: over millions of events the transcendentals'
+    # last-bit differences separate them).
+    cfg = SampleConfig(layers=[Layer(1.0, 0.0, 1.0, 0.0, 20000.0)], R=Sphere(False, angle=0.0),
+                       T=Sphere(False, angle=0.0))
+    n, limit = 1 << 14, 1 << 22
+    r, pp = engine.simulate_packets(cfg, n, seed=5, fp64=fp64)
+    dropped = np.all(pp[:, :5] == 0, axis=1)
+    print("dropped", int(dropped.sum()), "of", n, "longest kept history", int(pp[~dropped, 5].max()))
+    assert r.n_aborted == dropped.sum() > 0
+    assert np.all(pp[dropped, 5] == limit + 1)
+    assert np.all(pp[~dropped, 5] <= limit + 1)
+    assert np.allclose(pp[~dropped, :5].sum(1), 1.0)
+    b = engine.simulate_batch([cfg], [n], seed=5, streams=[0], chunk_packets=64, blocks_per_cu=1,
+                              fp64=fp64)[0]
+    assert b.n_aborted == r.n_aborted and b.sum == r.sum and b.events == r.events
+    o = O.simulate(spec_of(cfg), n, seed=5, threads=16, flags=0 if fp64 else O.FP32)
+    k1, k2 = r.n_aborted, o["n_aborted"]
+    print("oracle dropped", k2)
+    assert abs(k1 - k2) <= 4 * math.sqrt(k1 + k2) + 1
+
+
 @pytest.mark.parametrize("name,cfg", [("slab_between_slides", slab_between_slides()),
                                       ("three_layer", three_layer_sample())])
 def test_fp32_vs_fp64_kernel_deviation(engine, name, cfg):
