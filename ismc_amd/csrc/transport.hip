@@ -101,6 +101,20 @@ template <> struct GlassGuard<double> { static constexpr double value = 9.094947
 enum : int32_t { PH_NEED = 0, PH_ENTER = 1, PH_FLIGHT = 2, PH_DONE = 3, PH_SPHERE = 4, PH_END = 5,
                  PH_BOUND = 6 };
 
+// Constants of the interface a packet meets when it leaves a layer in one
+// direction (towards the neighbouring layer, or the ambient medium at the
+// stack's face), derived on the host in FP64: they depend on the two
+// indices only, so no hit recomputes them.
+template <typename R>
+struct alignas(16) Iface {
+  R eta;   // n_here / n_there (Snell)
+  R qq;    // ((n_there - n_here) / (n_there + n_here))^2, normal incidence
+  R cmin;  // entering a glass slide from a less dense layer: the least |mu_z|
+           // inside it (glass_entry_cos); 0 otherwise
+  R mut;   // mu_t here / mu_t there: rescales the remaining step s when both
+           // layers scatter
+};
+
 // Constants of one layer, derived on the host in FP64.
 template <typename R>
 struct alignas(16) Layer {
@@ -108,6 +122,7 @@ struct alignas(16) Layer {
   R absfrac, hg_a, hg_b, hg_c;  // mu_a/mu_t; 1+g^2; 1-g^2; 1-g
   R hg_d, hg_e, mu_t;        // 2g; 1/(2g); mu_t
   int32_t kind;
+  Iface<R> out[2];           // leaving towards lower z (0) / higher z (1)
 };
 
 // Everything a packet needs about its configuration.
@@ -121,7 +136,9 @@ struct alignas(16) Point {
   R roul_p, mu0_z;
   R cos_spec[2];             // |mu_z| >= cos_spec: specular (R) / unscattered (T)
   R p_return[2];             // sphere return probability; -1 without a sphere
-  R ent_n2[2], ent_n3[2];    // entrance: first layer index, and the one behind a slide
+  R ent_eta1[2], ent_qq1[2]; // entrance from the ambient medium into the first layer
+  R ent_eta2[2], ent_qq2[2]; // and from a first-layer slide into the layer behind it
+  R ent_cmin[2];             // glass_entry_cos bound of that entrance (0: none)
   Layer<R> layer[kMaxLayers];
 };
 
@@ -216,19 +233,20 @@ __device__ __forceinline__ R u01(uint32_t x) {
 // rp = (ni ct - nt ci)/(ni ct + nt ci), R = (rs^2 + rp^2)/2.  Same
 // near-normal shortcut (|mu_z| > 0.99999) and total-internal-reflection rule.
 // ct (the transmitted cosine, Snell) is returned for the refraction.
+// Both amplitudes are divided through by nt, so only eta = ni/nt and the
+// normal-incidence value qq = ((nt - ni)/(nt + ni))^2 enter; the host
+// derives both per interface.
 template <typename R>
-__device__ __forceinline__ R fresnel(R ni, R nt, R mu_z, R& ct) {
+__device__ __forceinline__ R fresnel(R eta, R qq, R mu_z, R& ct) {
   const R ci = DM<R>::abs(mu_z);
-  const R eta = ni * DM<R>::rcp(nt);
   const R st2 = eta * eta * ((R)1 - ci * ci);
   ct = DM<R>::sqrt(fmax((R)0, (R)1 - st2));
-  const R a = ni * ci, b = nt * ct, c = ni * ct, d = nt * ci;
-  const R rs = (a - b) * DM<R>::rcp(a + b);
-  const R rp = (c - d) * DM<R>::rcp(c + d);
-  const R q = (nt - ni) * DM<R>::rcp(nt + ni);
+  const R a = eta * ci, c = eta * ct;
+  const R rs = (a - ct) * DM<R>::rcp(a + ct);
+  const R rp = (c - ci) * DM<R>::rcp(c + ci);
   R Rf = (R)0.5 * (rs * rs + rp * rp);
   Rf = st2 >= (R)1 ? (R)1 : Rf;
-  Rf = ci > (R)0.99999 ? q * q : Rf;
+  Rf = ci > (R)0.99999 ? qq : Rf;
   return Rf;
 }
 
@@ -238,8 +256,8 @@ __device__ __forceinline__ R fresnel(R ni, R nt, R mu_z, R& ct) {
 template <typename R>
 __device__ __forceinline__ R entrance_R(const Point<R>& P, int side, R mu_z, R& ct1) {
   R ct2;
-  const R r1 = fresnel<R>(P.n_amb, P.ent_n2[side], mu_z, ct1);
-  const R r2 = fresnel<R>(P.ent_n2[side], P.ent_n3[side], mu_z, ct2);
+  const R r1 = fresnel<R>(P.ent_eta1[side], P.ent_qq1[side], mu_z, ct1);
+  const R r2 = fresnel<R>(P.ent_eta2[side], P.ent_qq2[side], mu_z, ct2);
   const R rg = r1 + ((R)1 - r1) * ((R)1 - r1) * r2 * DM<R>::rcp((R)1 - r1 * r2);
   return P.ent_glass[side] ? rg : r1;
 }
@@ -263,13 +281,13 @@ __device__ __forceinline__ R spin_z(R mu_z, R cos_t, R u_phi) {
   return DM<R>::abs(mu_z) > (R)0.99999 ? (mu_z > (R)0 ? cos_t : -cos_t) : general;
 }
 
-// cosine inside a glass slide of index ng, entered from index na, clamped
-// away from the critical angle of the way back (see GlassGuard)
+// cosine inside a glass slide of index ng entered from index na < ng,
+// clamped away from the critical angle of the way back (see GlassGuard):
+// at least cmin = sqrt(1 - (1 - guard) (na/ng)^2), which the host derives
+// per interface (Iface::cmin, Point::ent_cmin; 0 where no bound applies)
 template <typename R>
-__device__ __forceinline__ R glass_entry_cos(R ct, R na, R ng) {
-  const R q = na * DM<R>::rcp(ng);
-  const R cmin = DM<R>::sqrt(fmax((R)0, (R)1 - ((R)1 - GlassGuard<R>::value) * q * q));
-  return ng > na ? fmax(ct, cmin) : ct;
+__device__ __forceinline__ R glass_entry_cos(R ct, R cmin) {
+  return fmax(ct, cmin);
 }
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
@@ -339,8 +357,8 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
   uint32_t ev = 0, stream = 0;
   uint32_t e_int = 0, e_bnd = 0, e_inc = 0, e_sph = 0, e_rei = 0;
   // current layer cache
-  R L_zlo = 0, L_zhi = 0, L_n = 1, L_step = 0, L_abs = 0, L_ha = 0, L_hb = 0, L_hc = 0,
-    L_hd = 0, L_he = 0, L_mut = 0;
+  R L_zlo = 0, L_zhi = 0, L_step = 0, L_abs = 0, L_ha = 0, L_hb = 0, L_hc = 0, L_hd = 0,
+    L_he = 0;
   int L_kind = KIND_GLASS;
 
   for (;;) {
@@ -514,10 +532,9 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       s = glass ? (R)0 : s - d;
       const int nxt = muz > (R)0 ? layer + 1 : layer - 1;
       const bool outside = (nxt < 0) || (nxt >= P.n_layers);
-      const int nxc = outside ? layer : nxt;
-      const R nn = outside ? P.n_amb : P.layer[nxc].n;
+      const Iface<R> f = P.layer[layer].out[muz > (R)0 ? 1 : 0];
       R ct;
-      const R Rf = fresnel<R>(L_n, nn, muz, ct);
+      const R Rf = fresnel<R>(f.eta, f.qq, muz, ct);
       e_bnd += 1;
       phase = PH_FLIGHT;
       if (bu0 < Rf) {
@@ -528,13 +545,15 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
           side = nxt < 0 ? 0 : 1;
           phase = PH_SPHERE;  // su1, su2 hold the hit's return draw and new |mu_z|
         } else {
+          // f.cmin is 0 unless the next layer is a denser slide, and f.mut
+          // is 1 unless both layers scatter (in a slide s is 0 anyway)
           const Layer<R>& NL = P.layer[nxt];
-          if (NL.kind == KIND_GLASS) muz = DM<R>::csign(glass_entry_cos<R>(ct, L_n, NL.n), muz);
-          if (NL.kind != KIND_GLASS && s != (R)0) s *= L_mut * DM<R>::rcp(NL.mu_t);
+          muz = DM<R>::csign(glass_entry_cos<R>(ct, f.cmin), muz);
+          s *= f.mut;
           layer = nxt;
-          L_zlo = NL.zlo; L_zhi = NL.zhi; L_n = NL.n; L_step = NL.step_k; L_abs = NL.absfrac;
+          L_zlo = NL.zlo; L_zhi = NL.zhi; L_step = NL.step_k; L_abs = NL.absfrac;
           L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
-          L_mut = NL.mu_t; L_kind = NL.kind;
+          L_kind = NL.kind;
         }
       }
       // a packet still running after kMaxEventsPerPacket events is dropped
@@ -613,10 +632,10 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         muz = DM<R>::csign(ct, muz);
         layer = P.first_layer[side];
         const Layer<R>& NL = P.layer[layer];
-        L_zlo = NL.zlo; L_zhi = NL.zhi; L_n = NL.n; L_step = NL.step_k; L_abs = NL.absfrac;
+        L_zlo = NL.zlo; L_zhi = NL.zhi; L_step = NL.step_k; L_abs = NL.absfrac;
         L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
-        L_mut = NL.mu_t; L_kind = NL.kind;
-        if (L_kind == KIND_GLASS) muz = DM<R>::csign(glass_entry_cos<R>(ct, P.n_amb, L_n), muz);
+        L_kind = NL.kind;
+        muz = DM<R>::csign(glass_entry_cos<R>(ct, P.ent_cmin[side]), muz);
         z = side ? P.t_tot : (R)0;
         s = L_kind == KIND_GLASS ? (R)0 : DM<R>::nlog2(u1) * L_step;
         phase = PH_FLIGHT;
@@ -654,8 +673,10 @@ __global__ void eval_kernel(int op, const Point<R>* pt, const double* in4, int n
   const double* x = in4 + 4 * i;
   double o0 = 0, o1 = 0;
   if (op == 0) {
+    // the interface constants as the host derives them (FP64, then R)
     R ct;
-    o0 = (double)fresnel<R>((R)x[0], (R)x[1], (R)x[2], ct);
+    const double q = (x[1] - x[0]) / (x[1] + x[0]);
+    o0 = (double)fresnel<R>((R)(x[0] / x[1]), (R)(q * q), (R)x[2], ct);
     o1 = (double)ct;
   } else if (op == 1) {
     const R g = (R)x[0];
@@ -831,6 +852,27 @@ static int validate(ismc_ctx* c, const ismc_config* cfg) {
   return 0;
 }
 
+// Constants of the interface from index na (a layer of extinction ta, or
+// the ambient medium) to index nb (a layer of extinction tb, a slide, or
+// the ambient medium), in FP64 (the same expressions and order as the
+// oracle's, so the FP64 kernel draws its decisions from equal values).
+template <typename R>
+static Iface<R> make_iface(double na, double nb, bool a_turbid, double ta, bool b_turbid,
+                           bool b_glass, double tb) {
+  Iface<R> f;
+  f.eta = (R)(na / nb);
+  const double q = (nb - na) / (nb + na);
+  f.qq = (R)(q * q);
+  double cmin = 0.0;
+  if (b_glass && nb > na) {
+    const double r = na / nb;
+    cmin = std::sqrt(std::max(0.0, 1.0 - (1.0 - (double)GlassGuard<R>::value) * r * r));
+  }
+  f.cmin = (R)cmin;
+  f.mut = (a_turbid && b_turbid) ? (R)(ta / tb) : (R)1;
+  return f;
+}
+
 template <typename R>
 static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
   memset(P, 0, sizeof(*P));
@@ -859,6 +901,17 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
       Ly.mu_t = (R)mu_t;
     }
   }
+  const double namb = cfg->n_ambient;
+  for (int i = 0; i < L; ++i) {
+    const bool turbid = cfg->mu_s[i] != 0.0;
+    for (int dir = 0; dir < 2; ++dir) {
+      const int j = dir ? i + 1 : i - 1;
+      const bool inside = j >= 0 && j < L;
+      P->layer[i].out[dir] = make_iface<R>(cfg->n[i], inside ? cfg->n[j] : namb, turbid, cfg->mu_t[i],
+                                           inside && cfg->mu_s[j] != 0.0, inside && cfg->mu_s[j] == 0.0,
+                                           inside ? cfg->mu_t[j] : 1.0);
+    }
+  }
   P->t_tot = (R)z;
   P->n_amb = (R)cfg->n_ambient;
   P->roul_thr = (R)cfg->roulette_threshold;
@@ -871,8 +924,15 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
   for (int k = 0; k < 2; ++k) {
     P->first_layer[k] = first[k];
     P->ent_glass[k] = cfg->mu_s[first[k]] == 0.0;
-    P->ent_n2[k] = (R)cfg->n[first[k]];
-    P->ent_n3[k] = (behind[k] >= 0 && behind[k] < L) ? (R)cfg->n[behind[k]] : (R)cfg->n_ambient;
+    {
+      const double n2 = cfg->n[first[k]];
+      const double n3 = (behind[k] >= 0 && behind[k] < L) ? cfg->n[behind[k]] : namb;
+      const bool slide = cfg->mu_s[first[k]] == 0.0;
+      const Iface<R> e1 = make_iface<R>(namb, n2, false, 1.0, false, slide, 1.0);
+      const Iface<R> e2 = make_iface<R>(n2, n3, false, 1.0, false, false, 1.0);
+      P->ent_eta1[k] = e1.eta; P->ent_qq1[k] = e1.qq; P->ent_cmin[k] = e1.cmin;
+      P->ent_eta2[k] = e2.eta; P->ent_qq2[k] = e2.qq;
+    }
     if (sp[k]->present) {
       P->cos_spec[k] = (R)std::sqrt(1.0 - sp[k]->fp);
       P->p_return[k] = (R)((sp[k]->pw * sp[k]->fs) / (1.0 - sp[k]->pw * (1.0 - sp[k]->f)));

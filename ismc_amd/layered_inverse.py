@@ -150,31 +150,45 @@ class LayeredCost:
 
 
 def solve(problems: Sequence[LayeredProblem], guesses, packets=1_000_000, step=0.5, tol=1e-3,
-          max_evals=2000, restarts=0, seed=1234, stream_base=0, engine: Optional[Engine] = None,
-          log=None):
+          max_evals=2000, restarts=0, restart_rtol=None, seed=1234, stream_base=0,
+          engine: Optional[Engine] = None, log=None):
     """PRAXIS per problem in log coordinates from `guesses` (linear units),
     all problems in lock step.  With `restarts`, PRAXIS starts again from
     its own result (fresh directions, step `step`, the same cost surface)
-    that many times: in three or more dimensions a long narrow valley of
-    the MC cost can stop it early.  Returns (solutions [P, n] linear, costs, evaluations
-    per solve, stats)."""
+    up to that many times: in three or more dimensions a long narrow valley
+    of the MC cost can stop it early.  With `restart_rtol`, a problem is
+    started again only while its last run lowered its cost by more than
+    that fraction, so restarts go on while they make progress and stop
+    where they do not.  Returns (solutions [P, n] linear, costs,
+    evaluations per solve, stats)."""
     eng = engine or default_engine()
     if len({len(p.unknowns) for p in problems}) != 1:
         raise ValueError("all problems of one batch need the same number of unknowns")
     cost = LayeredCost(eng, problems, packets, seed=seed,
                        streams=np.arange(len(problems)) + stream_base)
-    starts = [np.log(np.asarray(g, dtype=np.float64)) for g in guesses]
+    x = [np.log(np.asarray(g, dtype=np.float64)) for g in guesses]
+    f = np.full(len(problems), np.inf)
     on_round = None
     if log:
         on_round = lambda r, pending: (log("    round %d: %d solves pending" % (r, pending))
                                        if r % 20 == 0 else None)
     nev = np.zeros(len(problems), dtype=np.int64)
+    active = list(range(len(problems)))
     for _ in range(restarts + 1):
-        res = _praxis.minimize_batch(starts, cost, tol, step, max_evals=max_evals, on_round=on_round)
-        starts = [r[0] for r in res]
-        nev += np.array([r[2] for r in res])
-    sol = np.array([np.exp(r[0]) for r in res])
-    return sol, np.array([r[1] for r in res]), nev, {"launches": cost.launches, "packets": cost.packets}
+        if not active:
+            break
+        # PRAXIS numbers the runs it is given 0..k-1; the cost needs the problems'
+        run = list(active)
+        res = _praxis.minimize_batch([x[i] for i in run], lambda pts, ids: cost(pts, [run[k] for k in ids]),
+                                     tol, step, max_evals=max_evals, on_round=on_round)
+        active = []
+        for i, (xi, fi, ni) in zip(run, res):
+            nev[i] += ni
+            if restart_rtol is None or fi < f[i] * (1.0 - restart_rtol):
+                active.append(i)
+            x[i], f[i] = xi, fi
+    sol = np.array([np.exp(xi) for xi in x])
+    return sol, f.copy(), nev, {"launches": cost.launches, "packets": cost.packets}
 
 
 def measure(problem: LayeredProblem, layers: Sequence[Layer], packets=10_000_000, seed=99, stream=0,

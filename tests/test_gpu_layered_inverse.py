@@ -26,10 +26,11 @@ def test_two_layer_three_unknowns(engine):
         p.measured = LI.measure(p, tr.layers, packets=10_000_000, stream=500 + k, engine=engine)
         probs.append(p)
     guesses = [p.truth_of(tr.layers) * [1.5, 0.7, 1.4] for p, tr in zip(probs, truths)]
-    # one restart from PRAXIS's own result: on the MC cost a single run can
-    # stop early in a narrow valley (as the reference's own runs can)
-    sol, cost, nev, st = LI.solve(probs, guesses, packets=1_000_000, step=0.3, tol=1e-3, restarts=1,
-                                  engine=engine)
+    # PRAXIS restarts from its own result while a restart still lowers the
+    # cost by more than 10 %: on the MC cost a single run can stop early in
+    # a narrow valley (as the reference's own runs can)
+    sol, cost, nev, st = LI.solve(probs, guesses, packets=1_000_000, step=0.3, tol=1e-3, restarts=8,
+                                  restart_rtol=0.1, engine=engine)
     for p, tr, x, c, n in zip(probs, truths, sol, cost, nev):
         t = p.truth_of(tr.layers)
         print("truth", np.round(t, 4), "recovered", np.round(x, 4), "cost %.2e" % c, "evaluations", n)

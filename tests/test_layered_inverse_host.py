@@ -79,3 +79,31 @@ def test_three_unknowns_recovered_in_one_batch():
     # every round of pending evaluations is one launch
     assert st["launches"] == len(eng.launches) < nev.sum()
     assert max(eng.launches) <= 3 * len(probs)
+
+
+def test_restarts_continue_only_while_they_make_progress():
+    # Loose runs (tol 1e-2) stop short on the analytic readings; restarts
+    # from their own result refine them.  With restart_rtol a problem is
+    # restarted only while its last run lowered its cost by > that fraction,
+    # so the problems leave the batch at different restarts: the cost must
+    # still see each point under its own problem's id.
+    eng = AnalyticStackEngine()
+    truths = [stack(), stack(0.2, 8.0, 4.0, 30.0), stack(1.5, 8.0, 0.7, 9.0)]
+    probs = []
+    for tr in truths:
+        p = LI.LayeredProblem(tr, UNK, {"R_d": 0, "T_d": 0, "R_d_back": 0, "T_u": 0})
+        p.measured = LI.measure(p, tr.layers, engine=eng)
+        probs.append(p)
+    starts = [p.truth_of(tr.layers) * f for p, tr, f in zip(probs, truths, ([1.4, 0.7, 1.3], [1.02, 1.0, 1.0],
+                                                                             [0.6, 1.5, 1.8]))]
+    one = LI.solve(probs, starts, packets=1000, step=0.3, tol=1e-2, engine=eng)
+    many = LI.solve(probs, starts, packets=1000, step=0.3, tol=1e-2, restarts=10, restart_rtol=0.1, engine=eng)
+    fixed = LI.solve(probs, starts, packets=1000, step=0.3, tol=1e-2, restarts=10, engine=eng)
+    assert np.all(many[1] <= one[1])
+    assert np.all(many[2] > one[2]) and many[2].sum() < fixed[2].sum()
+    # problems leave at different restarts (evaluation counts differ)
+    assert len(set(many[2].tolist())) > 1
+    # each reported cost is the cost of its own solution
+    cost = LI.LayeredCost(eng, probs, 1000, streams=np.arange(len(probs)))
+    again = cost([np.log(x) for x in many[0]], list(range(len(probs))))
+    assert np.allclose(again, many[1])
