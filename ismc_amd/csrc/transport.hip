@@ -10,11 +10,12 @@
 // MLMC.py:76-202 of the reference; the structure is our own:
 //
 //  * Event machine.  Each loop iteration advances every lane in flight by
-//    one event of its packet: a flight that ends in an interaction (hop,
-//    drop, roulette, spin) or on an interface.  A lane that reaches an
-//    interface records its event's draws and waits; the interface code
-//    (Fresnel reflect / refract / exit, layer change) runs for all waiting
-//    lanes at once when enough of them wait (the bound-lane threshold).
+//    several events of its packet, one per flight step: a flight that ends
+//    in an interaction (hop, drop, roulette, spin) or on an interface.  A
+//    lane that reaches an interface records its event's draws and waits;
+//    the interface code (Fresnel reflect / refract / exit, layer change)
+//    runs for all waiting lanes at once, before a flight step, when enough
+//    of them wait (the bound-lane threshold).
 //    The rare steps (an entrance through the outer face, a sphere
 //    decision, the end-of-packet tally, taking a new packet) likewise wait
 //    for the idle-lane threshold and run together in one "rare pass".
@@ -70,14 +71,17 @@ constexpr int kMaxLayers = ISMC_MAX_LAYERS;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr uint32_t kMaxEventsPerPacket = 1u << 22;
-// Flight events per lane per loop iteration: in the FP32 kernel a second
-// one amortizes the iteration's bookkeeping (ballots, pass decisions);
-// lanes that stopped at an interface, a sphere or their end sit it out.
-// Measured on MI355X: +4 % (slab between slides), +6 % (three layers) over
-// one; the FP64 kernel, at 3 waves per SIMD, loses 4 % with two.
-template <typename R> struct FlightSteps { static constexpr int value = 1; };
-template <> struct FlightSteps<float> { static constexpr int value = 2; };
-// waves per SIMD the FP32 tally kernel is compiled for (72 VGPRs: no
+// Flight steps per loop iteration.  More steps share one iteration's rare
+// pass decision and ballots between more events; the interface pass runs
+// before every step, so a lane that stopped at an interface in one step
+// flies again in the next once enough lanes wait.  Measured on MI355X
+// (slab between slides / three layers / one layer mu_s 10, FP32): 4 steps
+// +4.5 % / +3.8 % / +12 % over 2, 6 steps a further +0.7 % / +0.8 % /
+// -0.4 %, 8 steps no better on the slab and -7 % on mu_s 10; FP64 with 3
+// steps +4 % over 1.
+template <typename R> struct FlightSteps { static constexpr int value = 3; };
+template <> struct FlightSteps<float> { static constexpr int value = 6; };
+// waves per SIMD the FP32 tally kernel is compiled for (66 VGPRs: no
 // spills); the FP64 kernel needs about twice the registers and the
 // per-packet output variant a few more, both are left to the compiler
 template <typename R, bool kPerPacket> struct MinWaves { static constexpr int value = 1; };
@@ -362,12 +366,12 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
   int L_kind = KIND_GLASS;
 
   for (;;) {
-    // Lanes in flight run their next event every iteration.  A flight that
-    // reaches an interface only records its event's draws and waits; the
-    // interface code (Fresnel, refraction, layer change) runs in an
-    // "interface pass" once bound_thr lanes wait (or none is in flight),
-    // after which those lanes go on with their next flight event in the
-    // same iteration.  The rare steps (sphere decision, end-of-packet tally,
+    // Lanes in flight run their next event every flight step.  A flight
+    // that reaches an interface only records its event's draws and waits;
+    // the interface code (Fresnel, refraction, layer change) runs in an
+    // "interface pass", before a flight step, once bound_thr lanes wait (or
+    // none is in flight), after which those lanes go on with their next
+    // flight event in that step.  The rare steps (sphere decision, end-of-packet tally,
     // refill, entrance) likewise run in a "rare pass" once idle_thr lanes
     // wait for one.  So the wave does not pay for the code of every path on
     // every iteration, and every packet still draws the same Philox block
@@ -393,7 +397,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     const uint64_t idle = __ballot(phase != PH_FLIGHT && phase != PH_BOUND && phase != PH_DONE);
     if ((fly | bnd | idle) == 0) break;
     const bool rare = idle != 0 && ((fly | bnd) == 0 || (uint32_t)__popcll(idle) >= idle_thr);
-    const bool bpass = bnd != 0 && (fly == 0 || (uint32_t)__popcll(bnd) >= bound_thr);
+    const bool bpass0 = bnd != 0 && (fly == 0 || (uint32_t)__popcll(bnd) >= bound_thr);
 
     if (rare) {
       // ---- sphere capture / return (CUDAISMLMC.h:377-410, 442-475, 518-550, 583-615)
@@ -520,7 +524,20 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       }
     }
 
+    // ---------------- flight steps: an interface pass if enough lanes wait
+    // at an interface, then one event per lane in flight (and, in the first
+    // step after a rare pass, per entering lane); a lane that reaches a
+    // sphere or its end stops for this iteration
+#pragma unroll
+    for (int step = 0; step < FlightSteps<R>::value; ++step) {
+    if (step > 0 && near_limit) break;
     const Point<R>& P = pts[set];
+    bool bpass = bpass0;  // decided with this iteration's first ballots
+    if (step > 0) {
+      const uint64_t f = __ballot(phase == PH_FLIGHT);
+      const uint64_t b = __ballot(phase == PH_BOUND);
+      bpass = b != 0 && (f == 0 || (uint32_t)__popcll(b) >= bound_thr);
+    }
     // ---------------- interface pass: the hits recorded since the last one
     // (CUDAISMLMC.h:180-257), with the draws of the hit's own event
     if (bpass && phase == PH_BOUND) {
@@ -564,12 +581,6 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       }
     }
 
-    // ---------------- events: FlightSteps per lane in flight (and, in a rare
-    // pass, one per entering lane); a lane that reaches an interface, a
-    // sphere or its end stops for this iteration
-#pragma unroll
-    for (int step = 0; step < FlightSteps<R>::value; ++step) {
-    if (step > 0 && near_limit) break;
     const bool run_flight = (phase == PH_FLIGHT);
     const bool run_enter = step == 0 && rare && (phase == PH_ENTER);
     if (__ballot(run_flight || run_enter) == 0) break;
