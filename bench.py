@@ -57,14 +57,18 @@ def cpu_baseline(seconds):
     cores = len(os.sched_getaffinity(0))
     threads = min(cores, 16)
     m = 2_000_000
-    t0 = time.perf_counter()
-    oracle.simulate(spec, m, seed=1, threads=threads)
-    c_rate = m / (time.perf_counter() - t0)
+    rates = {}
+    for name, flags in (("fp64", 0), ("fp32", oracle.FP32)):
+        t0 = time.perf_counter()
+        oracle.simulate(spec, m, seed=1, threads=threads, flags=flags)
+        rates[name] = m / (time.perf_counter() - t0)
     return {"value": py_rate, "unit": "packets/s", "cores": 1, "kind": "port",
             "sample": "%d packets of the pure-Python restatement (ISMLMC.py/MLMC.py structure, "
                       "math+random) of the same slab-between-slides workload" % n,
-            "c_oracle": {"value": c_rate, "unit": "packets/s", "cores": threads,
-                         "sample": "%d packets, C oracle FP64, OpenMP" % m}}
+            "c_oracle": {"value": rates["fp64"], "unit": "packets/s", "cores": threads,
+                         "sample": "%d packets, C oracle FP64, OpenMP" % m},
+            "c_oracle_fp32": {"value": rates["fp32"], "unit": "packets/s", "cores": threads,
+                              "sample": "%d packets, C oracle FP32, OpenMP" % m}}
 
 
 def cpu_inverse_baseline(packets_per_eval=2000, target_packets=1_000_000):
