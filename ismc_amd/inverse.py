@@ -130,9 +130,15 @@ def surrogate_seeds(sur, R_d, T_d, g=G, n=N_INDEX, t=THICK, grid=128, fine=16, p
     T_d = np.asarray(T_d, dtype=np.float64)
     Tf = np.maximum(T_d, 1e-6)
 
+    def predict(la_, ls_):
+        # log10 coordinates in; predict_log10 where the model has it
+        if hasattr(sur, "predict_log10"):
+            return sur.predict_log10(la_, ls_, g, n, t)
+        return sur.predict(10 ** la_, 10 ** ls_, g, n, t)
+
     def best_of(cand_a, cand_s, rows):
         # cand_*: [m, c] log10 coordinates of c candidates for each of m rows
-        pr = sur.predict(10 ** cand_a.ravel(), 10 ** cand_s.ravel(), g, n, t)
+        pr = predict(cand_a.ravel(), cand_s.ravel())
         pR = pr["R_d"].reshape(cand_a.shape)
         pT = pr["T_d"].reshape(cand_a.shape)
         c = ((pR - R_d[rows, None]) / R_d[rows, None]) ** 2 + ((pT - T_d[rows, None]) / Tf[rows, None]) ** 2
@@ -143,7 +149,7 @@ def surrogate_seeds(sur, R_d, T_d, g=G, n=N_INDEX, t=THICK, grid=128, fine=16, p
     la = np.linspace(-2, 1, grid)
     ls = np.linspace(0, 3, grid)
     A, Sg = np.meshgrid(la, ls, indexing="ij")
-    pred = sur.predict(10 ** A.ravel(), 10 ** Sg.ravel(), g, n, t)
+    pred = predict(A.ravel(), Sg.ravel())
     # every measurement against every cell, on the GPU
     search = argmin or (lambda pR, pT, R, T: grid_argmin(pR, pT, R, T, device=sur.mlp.device))
     k = search(pred["R_d"], pred["T_d"], R_d, T_d)

@@ -218,6 +218,22 @@ class Surrogate:
         p = expit(z)
         return {k: p[:, i] for i, k in enumerate(OUTPUTS)}
 
+    def predict_log10(self, log_mua, log_mus, g, n, t, outputs=("R_d", "T_d")):
+        """predict() for points given as log10 mu_a, log10 mu_s and scalar
+        g, n, t, with only `outputs` activated: the inputs are scaled
+        straight into float32 (no 10**x followed by log10) for the large
+        candidate grids of the inverse solver's seeds."""
+        la = np.asarray(log_mua, dtype=np.float64).ravel()
+        ls = np.asarray(log_mus, dtype=np.float64).ravel()
+        lo, hi = RANGES[:, 0], RANGES[:, 1]
+        x = np.empty((la.size, 5), dtype=np.float32)
+        x[:, 0] = 2.0 * (la - lo[0]) / (hi[0] - lo[0]) - 1.0
+        x[:, 1] = 2.0 * (ls - lo[1]) / (hi[1] - lo[1]) - 1.0
+        for j, v in ((2, g), (3, n), (4, t)):
+            x[:, j] = 2.0 * (float(v) - lo[j]) / (hi[j] - lo[j]) - 1.0
+        z = self.mlp.forward(x)
+        return {k: expit(z[:, OUTPUTS.index(k)]) for k in outputs}
+
     def save(self, path):
         np.savez(path, params=self.mlp.get_params(), arch=np.array(self.mlp.arch),
                  ranges=RANGES, info=np.array([repr(self.train_info)]))

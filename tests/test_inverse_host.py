@@ -110,3 +110,31 @@ def test_mc_newton_fallback_recovers_truth_from_far_starts():
     assert st["launches"] <= 30 and np.all(nev % 3 == 0)
     # every launch held three points per pending solve, and fewer as solves finished
     assert all(n % 3 == 0 for n in eng.calls) and eng.calls[-1] <= eng.calls[0]
+
+
+def test_predict_log10_equals_predict():
+    # the seeds' fast path (inputs built from log10 coordinates, two outputs)
+    # gives the network the same float32 inputs as predict()
+    from ismc_amd import surrogate
+
+    class Recording:
+        device = 0
+
+        def __init__(self):
+            self.seen = []
+
+        def forward(self, x):
+            x = np.asarray(x, dtype=np.float32)
+            self.seen.append(x.copy())
+            return np.stack([x[:, 0] - 0.5 * x[:, 1], x[:, 1] * x[:, 4] - x[:, 0], x[:, 2] + x[:, 3], x[:, 3]],
+                            1).astype(np.float32)
+
+    sur = surrogate.Surrogate(Recording(), {})
+    rng = np.random.default_rng(3)
+    la, ls = rng.uniform(-2, 1, 5000), rng.uniform(0, 3, 5000)
+    a = sur.predict(10 ** la, 10 ** ls, 0.8, 1.4, 0.1)
+    b = sur.predict_log10(la, ls, 0.8, 1.4, 0.1)
+    assert set(b) == {"R_d", "T_d"}
+    assert np.max(np.abs(sur.mlp.seen[0] - sur.mlp.seen[1])) <= 2 ** -22
+    for k in b:
+        assert np.allclose(a[k], b[k], rtol=0, atol=1e-6)
