@@ -374,11 +374,11 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     // the interface code (Fresnel, refraction, layer change) runs in an
     // "interface pass", before a flight step, once bound_thr lanes wait (or
     // none is in flight), after which those lanes go on with their next
-    // flight event in that step.  The rare steps (sphere decision, end-of-packet tally,
-    // refill, entrance) likewise run in a "rare pass" once idle_thr lanes
-    // wait for one.  So the wave does not pay for the code of every path on
-    // every iteration, and every packet still draws the same Philox block
-    // for the same event.
+    // flight event in that step.  The rare steps (sphere decision,
+    // end-of-packet tally, refill, entrance) likewise run in a "rare pass"
+    // at the top of an iteration once idle_thr lanes wait for one.  So the
+    // wave does not pay for the code of every path on every step, and every
+    // packet still draws the same Philox block for the same event.
     //
     // A packet in flight after more than kMaxEventsPerPacket events is
     // dropped (counted, not tallied) before it runs another.  An event
