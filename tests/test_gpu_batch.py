@@ -108,6 +108,25 @@ def test_adaptive_sweep_matches_per_point_plan(engine):
     assert info["total_packets"] == sum(need)
 
 
+def test_full_lookup_grid_sweep(engine):
+    # The 64 x 64 lookup grid at its full size (mu_a 0.01-10, mu_s 1-1000,
+    # adaptive 1e4-1e6 packets per point, 0.5 % RSE target), with
+    # size-independent checks: every point reaches the target or the cap,
+    # the packet total adds up, and sampled points equal single-point runs
+    # of the same packet ids on the same stream, bit for bit.
+    mua, mus, cfgs = lookup_grid(64, 64)
+    out, info = lookup.adaptive_sweep(cfgs, rse=0.005, pilot=10000, n_min=10000, n_max=1_000_000,
+                                      seed=4321, engine=engine, autotune=True)
+    n = np.array([o["n_packets"] for o in out])
+    rse = np.array([o["rse_R"] for o in out])
+    assert np.all((rse <= 0.0056) | (n == 1_000_000)) and np.all((n >= 10000) & (n <= 1_000_000))
+    assert info["total_packets"] == n.sum() > 4096 * 10000
+    for k in (0, 63, 2080, 4032, 4095):
+        one = engine.simulate(cfgs[k], int(n[k]), seed=4321, stream=k)
+        for t in one.mean:
+            assert one.sum[t] / one.n_packets == out[k]["mean"][t], (k, t)
+
+
 def test_run_lookup_from_ini(engine, tmp_path):
     f = ini_mod.load(os.path.join(HERE, "golden", "sample_lookup.ini"))
     out = lookup.run_lookup(f, seed=2, engine=engine, rng_seed=4)

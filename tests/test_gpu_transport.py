@@ -174,6 +174,19 @@ def test_full_size_step_is_additive_over_packet_ranges(engine):
     assert h[0].sum != h[1].sum  # the halves are different packets
 
 
+def test_three_layers_at_full_size(engine):
+    # three turbid layers, dual spheres, 1e8 packets in one launch:
+    # energy balance, no packet dropped, agreement with the oracle
+    cfg = three_layer_sample()
+    r = engine.simulate(cfg, 100_000_000, seed=1234)
+    assert r.n_aborted == 0 and r.n_packets == 100_000_000
+    assert abs(sum(r.mean.values()) - 1.0) < 2e-4
+    o = O.simulate(spec_of(cfg), 1_000_000, seed=4321, threads=16)
+    z = zscores(vec(r.mean), vec(r.se), vec(o["mean"]), vec(o["se"]))
+    print("three layers, 1e8 packets vs oracle 1e6: z", np.round(z, 2))
+    assert np.all(np.abs(z) < 3.0), np.round(z, 2)
+
+
 @pytest.mark.parametrize("fp64", [False, True], ids=["fp32", "fp64"])
 def test_event_limit_drops_runaway_packets(engine, fp64):
     # A matched, non-absorbing slab 20,000 mean free paths thick: about one
