@@ -100,3 +100,22 @@ def test_grid_argmin_equals_numpy(engine):
     assert np.array_equal(I.grid_argmin(pR2, pT, R[:64], T[:64], device=engine.device),
                           np_grid_argmin(pR2, pT, R[:64], T[:64]))
     assert np.all(I.grid_argmin(pR2, pT, R[:64], T[:64], device=engine.device) == 4000)
+
+
+def test_full_size_pipeline(engine):
+    # The inverse workload at its full size: 4096 synthetic measurements
+    # (1e6 packets, 1 % noise) through the surrogate's seeds, six batched MC
+    # defect-correction launches, the MC judge at 1e6 packets and the
+    # fallback on its misses.  Checked: the solutions fit the measurements
+    # as well as the truth does in most cases, the recovered scattering is
+    # accurate and the judge's misses are few and mostly repaired.
+    sur = Surrogate.load(packaged_weights())
+    r = I.run(sur, count=4096, methods=("newton",), with_ablation=False, plain_count=1, engine=engine,
+              log=lambda *a: None)
+    s = r["seeded_newton"]
+    print({k: s[k] for k in ("solves_per_s", "misses", "misses_repaired_by_fallback", "fits_as_well_as_truth")},
+          s["errors"])
+    assert s["mean_evals"] == 6 and s["launches"] == 6
+    assert s["fits_as_well_as_truth"] > 0.78
+    assert s["errors"]["median_mus_reduced"] < 0.025 and s["errors"]["median_mu_a"] < 0.12
+    assert s["misses"] < 100 and s["misses_repaired_by_fallback"] >= s["misses"] // 3
