@@ -29,8 +29,8 @@ import numpy as np
 # 0 = the library's default: 256 .. 1024 packets (2048 from 256 points),
 # 14 sqrt(packets per resident wave)
 CHUNKS = (0, 512, 1024, 2048, 4096, 8192, 16384, 65536)
-IDLE = (4, 8, 16, 32)
-BOUND = (4, 8, 16, 32)
+IDLE = (4, 8, 10, 16, 32)
+BOUND = (4, 8, 12, 16, 32)
 BLOCKS = (0, 2, 4)
 
 
@@ -100,11 +100,11 @@ class AutoTuner:
             table.append(t)
             return t
 
-        # start from the library's defaults (occupancy-limited grid, 8 idle
-        # lanes, 16 bound lanes)
+        # start from the library's defaults (occupancy-limited grid, 10 idle
+        # lanes, 12 bound lanes)
         best = {"blocks_per_cu": 0, "chunk_packets": chunks[0],
-                "idle_lanes": 8 if 8 in self.idle else self.idle[0],
-                "bound_lanes": 16 if 16 in self.bound else self.bound[0]}
+                "idle_lanes": 10 if 10 in self.idle else self.idle[0],
+                "bound_lanes": 12 if 12 in self.bound else self.bound[0]}
         for knob, values in (("chunk_packets", chunks), ("idle_lanes", self.idle),
                              ("bound_lanes", self.bound), ("blocks_per_cu", self.blocks)):
             rows = [timed(dict(best, **{knob: v})) for v in values]

@@ -49,7 +49,7 @@ _RT = [TALLIES.index("R_d"), TALLIES.index("T_d")]
 # packets hit an interface about twice per history, so lanes waiting at one
 # are served after 8 of them gather rather than 16.  Measured on MI355X,
 # 4096 points x 1e5 packets: 86.2 ms against 90.6 ms with the library's
-# defaults (8 idle, 16 bound lanes) and 87.0 with 4 bound lanes; 4096 x
+# defaults of the time (8 idle, 16 bound lanes) and 87.0 with 4 bound lanes; 4096 x
 # 1e6: 773 against 800 and 779 ms.  Every solver here uses it, the plain
 # one included.  Tallies do not depend on it.
 LAUNCH = {"idle_lanes": 8, "bound_lanes": 8}
