@@ -61,8 +61,13 @@ class AutoTuner:
 
     def __init__(self, engine, budget_packets: int = 1 << 26, chunks: Sequence[int] = CHUNKS,
                  blocks: Sequence[int] = BLOCKS, idle: Sequence[int] = IDLE,
-                 bound: Sequence[int] = BOUND, repeats: int = 2):
+                 bound: Sequence[int] = BOUND, repeats: int = 2, min_gain: float = 0.01):
         self.engine = engine
+        # a knob leaves its current value only for one measured faster by
+        # more than this fraction: timings of a bounded sample differ by
+        # about a percent from run to run, and a shape chosen on noise can
+        # be slower on the full workload
+        self.min_gain = float(min_gain)
         self.budget = int(budget_packets)
         self.chunks = tuple(int(c) for c in chunks)
         self.blocks = tuple(int(b) for b in blocks)
@@ -108,7 +113,10 @@ class AutoTuner:
         for knob, values in (("chunk_packets", chunks), ("idle_lanes", self.idle),
                              ("bound_lanes", self.bound), ("blocks_per_cu", self.blocks)):
             rows = [timed(dict(best, **{knob: v})) for v in values]
-            best[knob] = max(rows, key=lambda t: t["packets_per_s"])[knob]
+            top = max(rows, key=lambda t: t["packets_per_s"])
+            cur = [t for t in rows if t[knob] == best[knob]]
+            if not cur or top["packets_per_s"] > cur[0]["packets_per_s"] * (1.0 + self.min_gain):
+                best[knob] = top[knob]
         self.tables[signature(cfgs, n_packets, fp64)] = table
         return best
 

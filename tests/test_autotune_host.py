@@ -65,3 +65,20 @@ def test_signature_separates_glass_and_precision():
     assert autotune.signature(a, 1e6) != autotune.signature(b, 1e6)
     assert autotune.signature(a, 1e6) != autotune.signature(a, 1e6, fp64=True)
     assert autotune.signature(a, 1.1e6) == autotune.signature(a, 1.5e6)
+
+
+def test_knobs_stay_unless_a_shape_is_clearly_faster():
+    # a candidate measured only 0.5 % faster than the default does not move
+    # the knob; one 5 % faster does
+    class Close(TimedEngine):
+        def simulate_batch(self, cfgs, n_packets, seed=0, streams=None, fp64=False, blocks_per_cu=0,
+                           chunk_packets=0, idle_lanes=0, bound_lanes=0, **kw):
+            npk = np.broadcast_to(np.asarray(n_packets), (len(cfgs),))
+            ms = 1.0 - (0.005 if idle_lanes == 16 else 0.0) - (0.05 if bound_lanes == 16 else 0.0)
+            return [Result(n_packets=int(n), mean={}, se={}, sum={}, sumsq={}, events={}, kernel_ms=ms)
+                    for n in npk]
+    cfgs = [workloads.single_layer(0.1 * (k + 1), 10.0) for k in range(8)]
+    kw = autotune.AutoTuner(Close(), budget_packets=1 << 20).params(cfgs, 1 << 22)
+    assert kw["idle_lanes"] == 10 and kw["bound_lanes"] == 16
+    kw0 = autotune.AutoTuner(Close(), budget_packets=1 << 20, min_gain=0.0).params(cfgs, 1 << 22)
+    assert kw0["idle_lanes"] == 16
