@@ -95,9 +95,10 @@ typedef struct {
     int32_t chunk_packets;   /* packets per work-queue chunk */
     int32_t use_graph;       /* replay through a captured hipGraph when shapes repeat */
     int32_t idle_lanes;      /* lanes of a wave waiting for an entrance, sphere decision
-                                or new packet before the wave serves them (1..64) */
+                                or new packet before the wave serves them (1..64;
+                                default 10) */
     int32_t bound_lanes;     /* lanes of a wave waiting at an interface before the wave
-                                serves them (1..64) */
+                                serves them (1..64; default 12) */
     int32_t _reserved;
 } ismc_tune;
 
