@@ -190,7 +190,9 @@ def inverse_bench(eng, count):
     log = lambda *x: print(*x, file=sys.stderr, flush=True)
     out = {}
     for key, methods, kw in (
-            ("noise_1pct", ("newton", "praxis"), dict(count=count, plain_count=32, noise=0.01)),
+            # 96 plain solves (about 0.17 s each) at 1 % noise, where few of
+            # them recover both parameters within 1 %
+            ("noise_1pct", ("newton", "praxis"), dict(count=count, plain_count=96, noise=0.01)),
             ("noise_0.1pct", ("newton",), dict(count=max(count // 4, 64), plain_count=32, noise=0.001,
                                                 meas_packets=10_000_000, refine_packets=1_000_000,
                                                 check_packets=10_000_000, plain_packets=10_000_000,
