@@ -78,8 +78,8 @@ constexpr uint32_t kMaxEventsPerPacket = 1u << 22;
 // (slab between slides / three layers / one layer mu_s 10, FP32): 4 steps
 // +4.5 % / +3.8 % / +12 % over 2, 6 steps a further +0.7 % / +0.8 % /
 // -0.4 %, 8 steps no better on the slab and -7 % on mu_s 10; FP64 with 3
-// steps +4 % over 1.
-template <typename R> struct FlightSteps { static constexpr int value = 3; };
+// steps +4 % over 1, with 4 a further +0.1 % / +0.3 % / +3.4 % (6: no gain).
+template <typename R> struct FlightSteps { static constexpr int value = 4; };
 template <> struct FlightSteps<float> { static constexpr int value = 6; };
 // waves per SIMD the FP32 tally kernel is compiled for (66 VGPRs: no
 // spills); the FP64 kernel needs about twice the registers and the
