@@ -321,7 +321,8 @@ def main():
     if a.inverse and not a.no_extra:
         out["inverse"] = inverse_bench(eng, a.inverse)
     if a.copy_probe:
-        ms = eng.copy_probe(a.copy_probe)
+        from ismc_amd import profiling
+        ms = profiling.copy_probe(eng.device, a.copy_probe)
         out["copy_probe"] = {"bytes": a.copy_probe, "kernel_ms": ms,
                              "GB_per_s": 2 * a.copy_probe / ms / 1e6}
     if not a.no_cpu:

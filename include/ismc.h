@@ -124,7 +124,11 @@ int ismc_device_info_get(ismc_ctx* ctx, ismc_device_info* out);
 int ismc_validate_config(const ismc_config* cfg, char* msg, int32_t msg_len);
 
 /* One configuration, packets first_packet_id .. first_packet_id+n-1 of
- * Philox stream (seed, stream).  Results do not depend on launch shape. */
+ * Philox stream (seed, stream).  Results do not depend on launch shape.
+ * At most 2^31 packets per configuration per call (ISMC_ERANGE beyond:
+ * the 64-bit fixed-point tallies, 2^-32 per unit of weight, keep twice
+ * the headroom that many packets need); split larger runs into
+ * packet-id ranges, whose tallies add exactly. */
 int ismc_simulate(ismc_ctx* ctx, const ismc_config* cfg, uint64_t n_packets, uint64_t seed,
                   uint64_t first_packet_id, uint32_t stream, uint32_t flags, ismc_tally* out);
 
@@ -158,11 +162,6 @@ int ismc_graph_stats(ismc_ctx* ctx, int64_t* captures, int64_t* launches);
  * (side R if a > 0 else T); op 5 spin: mu_z'(mu_z=a, g=b, u_hg=c, u_phi=d). */
 int ismc_eval_physics(ismc_ctx* ctx, int32_t op, const ismc_config* cfg, const double* in4,
                       int32_t n, uint32_t flags, double* out2);
-
-/* Copy `bytes` bytes between two device buffers with a plain streaming
- * kernel and return its device time: a known-size reference for the
- * hardware byte counters (FETCH_SIZE / WRITE_SIZE) of a profile run. */
-int ismc_copy_probe(ismc_ctx* ctx, int64_t bytes, float* kernel_ms);
 
 /* ---------------------------------------------------------------------
  * Surrogate MLP (no counterpart in the reference: it seeds the inverse
@@ -221,20 +220,6 @@ const char* ismc_table_last_error(void);
  * ismc_table_last_error. */
 int ismc_grid_argmin(int device, const float* pR, const float* pT, int32_t G, const double* R,
                      const double* T, int32_t M, int32_t* idx);
-
-/* ---------------------------------------------------------------------
- * VALU issue costs (no counterpart in the reference: the measurement
- * behind the transport kernel's roofline).  Op k (0 .. ismc_issue_rate_ops()
- * - 1, named by ismc_issue_rate_name) runs one instruction kind in 8
- * independent chains per lane, 64 workgroups of 256 lanes per CU, five
- * launches after a warm-up; returns wave64 instructions per second and the
- * shader clock the loops ran at (s_memtime over the 100 MHz real-time
- * counter).  cycles per instruction = 4 x CUs x clock / rate.
- * --------------------------------------------------------------------- */
-int32_t ismc_issue_rate_ops(void);
-const char* ismc_issue_rate_name(int32_t op);
-int ismc_issue_rate(int device, int32_t op, double* wave_insts_per_s, double* shader_ghz);
-const char* ismc_issue_rate_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -20,9 +20,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libismc.so"
 
 EXPORTS = ("ismc_create", "ismc_destroy", "ismc_last_error", "ismc_device_info_get",
            "ismc_simulate", "ismc_simulate_batch", "ismc_simulate_packets", "ismc_last_launch",
-           "ismc_eval_physics", "ismc_validate_config", "ismc_copy_probe", "ismc_graph_stats",
-           "ismc_table_uncertainty", "ismc_table_last_error", "ismc_issue_rate_ops",
-           "ismc_issue_rate_name", "ismc_issue_rate", "ismc_issue_rate_last_error", "ismc_grid_argmin")
+           "ismc_eval_physics", "ismc_validate_config", "ismc_graph_stats",
+           "ismc_table_uncertainty", "ismc_table_last_error", "ismc_grid_argmin")
 
 
 class Sphere(ctypes.Structure):
@@ -106,8 +105,6 @@ def load():
         L.ismc_validate_config.restype = ctypes.c_int
         L.ismc_graph_stats.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int64)]
         L.ismc_graph_stats.restype = ctypes.c_int
-        L.ismc_copy_probe.argtypes = [vp, ctypes.c_int64, P(ctypes.c_float)]
-        L.ismc_copy_probe.restype = ctypes.c_int
         L.ismc_table_uncertainty.argtypes = [ctypes.c_int] + [vp] * 6 + [ctypes.c_int32, ctypes.c_int32,
                                                                          ctypes.c_uint64, vp, vp]
         L.ismc_table_uncertainty.restype = ctypes.c_int
@@ -115,14 +112,6 @@ def load():
         L.ismc_table_last_error.restype = ctypes.c_char_p
         L.ismc_grid_argmin.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int32, vp]
         L.ismc_grid_argmin.restype = ctypes.c_int
-        L.ismc_issue_rate_ops.argtypes = []
-        L.ismc_issue_rate_ops.restype = ctypes.c_int32
-        L.ismc_issue_rate_name.argtypes = [ctypes.c_int32]
-        L.ismc_issue_rate_name.restype = ctypes.c_char_p
-        L.ismc_issue_rate.argtypes = [ctypes.c_int, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double)]
-        L.ismc_issue_rate.restype = ctypes.c_int
-        L.ismc_issue_rate_last_error.argtypes = []
-        L.ismc_issue_rate_last_error.restype = ctypes.c_char_p
         _lib = L
         return L
 

@@ -1,4 +1,5 @@
-"""Build libismc.so in-tree with hipcc for gfx950.
+"""Build libismc.so (the product: csrc/*.hip) and libismc_diag.so (the
+measurement probes: csrc/diag/*.hip) in-tree with hipcc for gfx950.
 
     python -m ismc_amd.build
 """
@@ -10,6 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libismc.so")
+DIAG_OUT = os.path.join(HERE, "libismc_diag.so")
 ARCH = os.environ.get("ISMC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -20,14 +22,20 @@ HIPCC_FLAGS = [
 ]
 
 
-def sources():
-    return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
+def sources(directory=CSRC):
+    return sorted(glob.glob(os.path.join(directory, "*.hip")) + glob.glob(os.path.join(directory, "*.cpp")))
 
 
 def build(verbose=False):
+    out = _link(sources(), OUT, verbose)
+    _link(sources(os.path.join(CSRC, "diag")), DIAG_OUT, verbose)
+    return out
+
+
+def _link(srcs, out, verbose):
     hipcc = os.environ.get("HIPCC", "hipcc")
-    tmp = OUT + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", tmp] + sources()
+    tmp = out + ".tmp"
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
@@ -37,8 +45,8 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed:\n" + msgs)
     if msgs and verbose:
         print(msgs)
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -1,4 +1,8 @@
-// VALU issue costs of single instruction kinds on the device (gfx950).
+// libismc_diag: measurement probes behind the transport kernel's roofline
+// and the byte counters of its profiles.  Not part of libismc (no
+// counterpart in the reference); declarations in include/ismc_diag.h.
+//
+// 1. VALU issue costs of single instruction kinds on the device (gfx950).
 //
 // Each kernel runs one instruction kind in 8 independent dependency chains
 // per lane (so issue, not latency, sets the pace at 8 waves per SIMD), over
@@ -14,6 +18,10 @@
 // math; 32-bit integer multiplies, mad_u64_u32 and FP64 adds take twice
 // that, and transcendentals (rcp, sqrt, log, exp, sin) four times.  The
 // three-input v_bitop3_b32 (Philox's key XORs) is measured as well.
+//
+// 2. A plain streaming copy of known size: a profile run reads what the
+// hardware byte counters (FETCH_SIZE / WRITE_SIZE) give for it, and scales
+// the transport kernel's counts by the same factor.
 #include <hip/hip_runtime.h>
 
 #include <array>
@@ -22,7 +30,7 @@
 #include <utility>
 #include <vector>
 
-#include "../../include/ismc.h"
+#include "../../../include/ismc_diag.h"
 
 namespace issue {
 
@@ -112,6 +120,11 @@ __global__ __launch_bounds__(256) void chain_kernel(uint32_t* out, unsigned long
   }
 }
 
+__global__ void copy_probe_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 using KernelFn = void (*)(uint32_t*, unsigned long long*, int, uint32_t);
 
 template <int... I>
@@ -138,21 +151,21 @@ const char* ismc_issue_rate_name(int32_t op) {
   return (op >= 0 && op < issue::N_OPS) ? issue::kOps[op].name : nullptr;
 }
 
-const char* ismc_issue_rate_last_error(void) { return issue::g_err.c_str(); }
+const char* ismc_diag_last_error(void) { return issue::g_err.c_str(); }
 
 int ismc_issue_rate(int device, int32_t op, double* wave_insts_per_s, double* shader_ghz) {
   using namespace issue;
   if (op < 0 || op >= N_OPS || !wave_insts_per_s || !shader_ghz) {
     g_err = "ismc_issue_rate: op outside 0..ismc_issue_rate_ops()-1 or null output";
-    return ISMC_EINVAL;
+    return ISMC_DIAG_EINVAL;
   }
   static const auto table = kernel_table(std::make_integer_sequence<int, N_OPS>{});
   const KernelFn kern = table[op];
   hipError_t e = hipSetDevice(device);
-  if (e != hipSuccess) return fail(ISMC_EDEVICE, "hipSetDevice", e);
+  if (e != hipSuccess) return fail(ISMC_DIAG_EDEVICE, "hipSetDevice", e);
   int n_cu = 0;
   if ((e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
-    return fail(ISMC_EDEVICE, "hipDeviceGetAttribute", e);
+    return fail(ISMC_DIAG_EDEVICE, "hipDeviceGetAttribute", e);
   const int blocks = n_cu * kBlocksPerCU;
   uint32_t* out = nullptr;
   unsigned long long* clk = nullptr;
@@ -164,28 +177,28 @@ int ismc_issue_rate(int device, int32_t op, double* wave_insts_per_s, double* sh
   double cyc = 0.0, rt = 0.0;
   if ((e = hipMalloc(&out, (size_t)blocks * 256 * sizeof(uint32_t))) != hipSuccess ||
       (e = hipMalloc(&clk, h.size() * sizeof(unsigned long long))) != hipSuccess) {
-    rc = fail(ISMC_ENOMEM, "hipMalloc", e);
+    rc = fail(ISMC_DIAG_ENOMEM, "hipMalloc", e);
     goto done;
   }
   if ((e = hipStreamCreate(&s)) != hipSuccess || (e = hipEventCreate(&e0)) != hipSuccess ||
       (e = hipEventCreate(&e1)) != hipSuccess) {
-    rc = fail(ISMC_EDEVICE, "hipStreamCreate/hipEventCreate", e);
+    rc = fail(ISMC_DIAG_EDEVICE, "hipStreamCreate/hipEventCreate", e);
     goto done;
   }
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, out, clk, kIters, 1u);  // warm-up
-  if ((e = hipEventRecord(e0, s)) != hipSuccess) { rc = fail(ISMC_EDEVICE, "hipEventRecord", e); goto done; }
+  if ((e = hipEventRecord(e0, s)) != hipSuccess) { rc = fail(ISMC_DIAG_EDEVICE, "hipEventRecord", e); goto done; }
   for (int r = 0; r < kLaunches; ++r)
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, out, clk, kIters, (uint32_t)(r + 2));
-  if ((e = hipGetLastError()) != hipSuccess) { rc = fail(ISMC_EDEVICE, "chain_kernel", e); goto done; }
+  if ((e = hipGetLastError()) != hipSuccess) { rc = fail(ISMC_DIAG_EDEVICE, "chain_kernel", e); goto done; }
   if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
       (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) {
-    rc = fail(ISMC_EDEVICE, "hipEvent timing", e);
+    rc = fail(ISMC_DIAG_EDEVICE, "hipEvent timing", e);
     goto done;
   }
   // clock of the last launch's loops
   if ((e = hipMemcpy(h.data(), clk, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost)) !=
       hipSuccess) {
-    rc = fail(ISMC_EDEVICE, "hipMemcpy", e);
+    rc = fail(ISMC_DIAG_EDEVICE, "hipMemcpy", e);
     goto done;
   }
   for (int b = 0; b < blocks; ++b) { cyc += (double)h[2 * b]; rt += (double)h[2 * b + 1]; }
@@ -197,6 +210,51 @@ done:
   if (s) (void)hipStreamDestroy(s);
   if (out) (void)hipFree(out);
   if (clk) (void)hipFree(clk);
+  return rc;
+}
+
+int ismc_copy_probe(int device, int64_t bytes, float* kernel_ms) {
+  using namespace issue;
+  if (bytes <= 0 || (bytes % 16) != 0 || !kernel_ms) {
+    g_err = "ismc_copy_probe: needs a positive multiple of 16 bytes and an output";
+    return ISMC_DIAG_EINVAL;
+  }
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(ISMC_DIAG_EDEVICE, "hipSetDevice", e);
+  int n_cu = 0;
+  if ((e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+    return fail(ISMC_DIAG_EDEVICE, "hipDeviceGetAttribute", e);
+  void* a = nullptr;
+  void* b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  const size_t n = (size_t)bytes / 16;
+  const int grid = n_cu * 8;
+  if ((e = hipMalloc(&a, (size_t)bytes)) != hipSuccess || (e = hipMalloc(&b, (size_t)bytes)) != hipSuccess) {
+    rc = fail(ISMC_DIAG_ENOMEM, "hipMalloc", e);
+    goto done;
+  }
+  if ((e = hipStreamCreate(&s)) != hipSuccess || (e = hipEventCreate(&e0)) != hipSuccess ||
+      (e = hipEventCreate(&e1)) != hipSuccess) {
+    rc = fail(ISMC_DIAG_EDEVICE, "hipStreamCreate/hipEventCreate", e);
+    goto done;
+  }
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, s, (const uint4*)b, (uint4*)a, n);  // warm
+  if ((e = hipEventRecord(e0, s)) != hipSuccess) { rc = fail(ISMC_DIAG_EDEVICE, "hipEventRecord", e); goto done; }
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, s, (const uint4*)a, (uint4*)b, n);
+  if ((e = hipGetLastError()) != hipSuccess) { rc = fail(ISMC_DIAG_EDEVICE, "copy_probe_kernel", e); goto done; }
+  if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+      (e = hipEventElapsedTime(kernel_ms, e0, e1)) != hipSuccess) {
+    rc = fail(ISMC_DIAG_EDEVICE, "hipEvent timing", e);
+    goto done;
+  }
+done:
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
   return rc;
 }
 

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -628,18 +629,32 @@ int ismc_mlp_gradients(ismc_mlp* p, const float* x, const float* y, int32_t B, f
 }
 
 int ismc_mlp_load_dataset(ismc_mlp* p, const float* x, const float* y, int64_t n) {
-  if (!p || !x || !y || n < p->max_batch) return ISMC_EINVAL;
+  if (!p || !x || !y || n < p->max_batch || n > INT32_MAX)
+    return mlp_err(p, ISMC_EINVAL, "dataset needs max_batch .. 2^31-1 rows and non-null arrays");
   MCHECK(p, hipSetDevice(p->device));
+  // the new buffers first: if one allocation fails the context keeps its
+  // previous dataset and index buffer intact
+  float* nx = nullptr;
+  float* ny = nullptr;
+  int* nidx = nullptr;
+  int rc = dalloc(p, &nx, (size_t)n * p->n_in);
+  rc = rc ? rc : dalloc(p, &ny, (size_t)n * p->n_out);
+  rc = rc ? rc : dalloc(p, &nidx, (size_t)n);
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipMemcpy(nx, x, sizeof(float) * n * p->n_in, hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess) e = hipMemcpy(ny, y, sizeof(float) * n * p->n_out, hipMemcpyHostToDevice);
+  if (rc || e != hipSuccess) {
+    if (nx) (void)hipFree(nx);
+    if (ny) (void)hipFree(ny);
+    if (nidx) (void)hipFree(nidx);
+    return rc ? rc : mlp_err(p, ISMC_EDEVICE, "dataset upload", e);
+  }
   if (p->ds_x) (void)hipFree(p->ds_x);
   if (p->ds_y) (void)hipFree(p->ds_y);
-  p->ds_x = p->ds_y = nullptr;
-  int rc;
-  if ((rc = dalloc(p, &p->ds_x, (size_t)n * p->n_in))) return rc;
-  if ((rc = dalloc(p, &p->ds_y, (size_t)n * p->n_out))) return rc;
   if (p->idx) (void)hipFree(p->idx);
-  if ((rc = dalloc(p, &p->idx, (size_t)n))) return rc;
-  MCHECK(p, hipMemcpy(p->ds_x, x, sizeof(float) * n * p->n_in, hipMemcpyHostToDevice));
-  MCHECK(p, hipMemcpy(p->ds_y, y, sizeof(float) * n * p->n_out, hipMemcpyHostToDevice));
+  p->ds_x = nx;
+  p->ds_y = ny;
+  p->idx = nidx;
   p->ds_n = n;
   return 0;
 }
@@ -654,6 +669,12 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
   int rc;
   if ((rc = check_batch(p, B))) return rc;
   if ((int64_t)steps_per_epoch * B > p->ds_n) return mlp_err(p, ISMC_EINVAL, "epoch larger than the dataset");
+  // every index must name a resident row: gather_batch reads ds_x / ds_y at
+  // them, so one outside 0 .. ds_n-1 would read beyond the dataset
+  const size_t n_idx = (size_t)epochs * steps_per_epoch * B;
+  for (size_t i = 0; i < n_idx; ++i)
+    if (perm[i] < 0 || (int64_t)perm[i] >= p->ds_n)
+      return mlp_err(p, ISMC_EINVAL, "perm holds an index outside the resident dataset");
   MCHECK(p, hipSetDevice(p->device));
   const int nb = (B + 255) / 256;
   std::vector<float> lp((size_t)nb * steps_per_epoch);

@@ -31,28 +31,11 @@
 #include <string>
 
 #include "../../include/ismc.h"
+#include "philox.hpp"
 
 namespace tablek {
 
 constexpr uint32_t kStreamTag = 0x7ab1eu;
-
-__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                       uint32_t k0, uint32_t k1, uint32_t out[4]) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
-    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
-    c1 = (uint32_t)p1;
-    c3 = (uint32_t)p0;
-    c0 = n0;
-    c2 = n2;
-    k0 += 0x9E3779B9u;
-    k1 += 0xBB67AE85u;
-  }
-  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
-}
 
 __device__ __forceinline__ double u01(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }
 
@@ -106,7 +89,7 @@ __global__ __launch_bounds__(256) void uncertainty_kernel(Args a) {
   const int pairs = (a.M + 1) / 2;
   for (int p = 0; p < pairs; ++p) {
     uint32_t w[4];
-    philox((uint32_t)cell, 0u, (uint32_t)p, kStreamTag, a.k0, a.k1, w);
+    ismc_rng::philox4x32_10((uint32_t)cell, 0u, (uint32_t)p, kStreamTag, a.k0, a.k1, w);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       if (2 * p + h >= a.M) break;

@@ -64,6 +64,7 @@
 #include <vector>
 
 #include "../../include/ismc.h"
+#include "philox.hpp"
 
 namespace ismc {
 
@@ -71,6 +72,11 @@ constexpr int kMaxLayers = ISMC_MAX_LAYERS;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr uint32_t kMaxEventsPerPacket = 1u << 22;
+// Packets of one point in one call.  A packet adds at most about 1 (its
+// weight, a little more after a roulette survival) to a tally in 2^-32
+// fixed point, so a u64 slot holds 2^31 packets with twice that headroom;
+// 2^32 packets of weight 1 would wrap it to 0.
+constexpr uint64_t kMaxPacketsPerPoint = 1ull << 31;
 // Flight steps per loop iteration.  More steps share one iteration's rare
 // pass decision and ballots between more events; the interface pass runs
 // before every step, so a lane that stopped at an interface in one step
@@ -198,30 +204,7 @@ template <> struct DM<double> {
   }
 };
 
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
-// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96 is three-input parity);
-// the compiler does not form it from two XORs by itself
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
-  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-
-__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                       uint32_t k0, uint32_t k1, uint32_t out[4]) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
-    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-    const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
-    c1 = (uint32_t)p1;
-    c3 = (uint32_t)p0;
-    c0 = n0;
-    c2 = n2;
-    k0 += 0x9E3779B9u;
-    k1 += 0xBB67AE85u;
-  }
-  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
-}
+using ismc_rng::philox4x32_10;
 
 // 23-bit uniform 1 - (x >> 9) 2^-23 in (0, 1], exact in float and double.
 // One v_alignbit_b32 shifts the word's top 23 bits under the exponent of
@@ -588,7 +571,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     const bool run_enter = step == 0 && rare && (phase == PH_ENTER);
     if (__ballot(run_flight || run_enter) == 0) break;
     uint32_t rw[4];
-    philox((uint32_t)pid, (uint32_t)(pid >> 32), ev, stream, k0, k1, rw);
+    philox4x32_10((uint32_t)pid, (uint32_t)(pid >> 32), ev, stream, k0, k1, rw);
     // The event's first three uniforms go straight into the registers an
     // interface hit or a reflected entrance keeps them in (bu0: Fresnel
     // draw; su1, su2: return draw and new |mu_z|).  Every lane that runs
@@ -666,11 +649,6 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (set_point0 != 0xffffffffu) flush_slot<R>(acc0, a.slots, set_point0, lane);
   if (set_point1 != 0xffffffffu) flush_slot<R>(acc1, a.slots, set_point1, lane);
-}
-
-__global__ void copy_probe_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
 }
 
 __global__ void zero_u64_kernel(unsigned long long* p, size_t n, uint32_t* counter) {
@@ -961,16 +939,16 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
   }
 }
 
+// Launch shape of a transport launch: workgroups per CU (the occupancy
+// limit unless the caller asks for a number) and packets per queue chunk.
 template <typename R>
-static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
-                      const uint64_t* first, const uint32_t* streams, int32_t n_points,
-                      uint64_t seed, uint32_t flags, const ismc_tune* tune, ismc_tally* out,
-                      double* per_packet_host) {
+static int launch_shape(ismc_ctx* c, const uint64_t* npk, int32_t n_points, const ismc_tune* tune,
+                        bool per_packet, int* per_cu_out, int* chunk_out) {
   const int bpc = (tune && tune->blocks_per_cu > 0) ? tune->blocks_per_cu : c->blocks_per_cu;
   // persistent grid: as many blocks as stay resident (occupancy), unless the
   // caller asks for a given number per CU
   int resident = 0;
-  if (per_packet_host)
+  if (per_packet)
     HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, true>, kBlock, 0));
   else
     HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, false>, kBlock, 0));
@@ -994,36 +972,97 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
     chunk = tune->chunk_packets;
   } else {
     uint64_t requested = 0;
-    for (int p = 0; p < n_points; ++p) requested += std::min<uint64_t>(npk[p], 1ull << 32);
+    for (int p = 0; p < n_points; ++p) requested += std::min<uint64_t>(npk[p], kMaxPacketsPerPoint);
     const uint64_t waves = (uint64_t)c->prop.multiProcessorCount * per_cu * kWavesPerBlock;
     const uint64_t fit = 14 * (uint64_t)std::sqrt((double)(requested / waves));
     const uint64_t ceiling = n_points >= 256 ? 2048 : 1024;
     chunk = (int)std::min<uint64_t>((uint64_t)c->chunk_packets,
                                     std::min<uint64_t>(ceiling, std::max<uint64_t>(256, fit)));
   }
-  // the queue: one range record per point; chunk numbers run point after point
+  *per_cu_out = per_cu;
+  *chunk_out = chunk;
+  return 0;
+}
+
+// The queue record of point p (chunk numbers run point after point) after
+// the range checks; *n_chunks and *total advance past it.
+static int point_range(ismc_ctx* c, int p, const uint64_t* npk, const uint64_t* first,
+                       const uint32_t* streams, int chunk, uint64_t* n_chunks, uint64_t* total,
+                       Range* r) {
+  if (npk[p] > kMaxPacketsPerPoint)
+    return set_err(c, ISMC_ERANGE, "point %s%lld: more than 2^31 packets in one call", "", p);
+  const uint64_t f0 = first ? first[p] : 0ull;
+  if (f0 + npk[p] < f0) return set_err(c, ISMC_ERANGE, "packet id range overflows%s");
+  if (*n_chunks > 0xffffffffull) return set_err(c, ISMC_ERANGE, "too many chunks%s");
+  r->first_pid = f0;
+  r->out_base = *total;
+  r->n_packets = npk[p];
+  r->chunk0 = (uint32_t)*n_chunks;
+  r->stream = streams ? streams[p] : (uint32_t)p;
+  *n_chunks += (npk[p] + (uint64_t)chunk - 1) / (uint64_t)chunk;
+  *total += npk[p];
+  return 0;
+}
+
+// Kernel arguments of a launch whose buffers are given.
+static Args make_args(ismc_ctx* c, const ismc_tune* tune, void* points, void* ranges, void* slots,
+                      void* counter, double* per_packet, int32_t n_points, uint64_t n_chunks,
+                      int chunk, uint64_t seed) {
+  Args a{};
+  a.points = points;
+  a.ranges = (const Range*)ranges;
+  a.n_points = (uint32_t)n_points;
+  a.n_chunks = (uint32_t)n_chunks;
+  a.chunk_packets = (uint32_t)chunk;
+  a.seed_lo = (uint32_t)seed;
+  a.seed_hi = (uint32_t)(seed >> 32);
+  a.chunk_counter = (uint32_t*)counter;
+  a.slots = (unsigned long long*)slots;
+  a.per_packet = per_packet;
+  a.idle_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->idle_lanes > 0) ? tune->idle_lanes
+                                                                                 : c->idle_lanes));
+  a.bound_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->bound_lanes > 0) ? tune->bound_lanes
+                                                                                   : c->bound_lanes));
+  return a;
+}
+
+// persistent grid: resident blocks, but no more than the packets fill
+static int grid_for(ismc_ctx* c, int per_cu, uint64_t total) {
+  const uint64_t waves_useful = std::max<uint64_t>(1, (total + 63) / 64);
+  const uint64_t blocks_useful = (waves_useful + kWavesPerBlock - 1) / kWavesPerBlock;
+  return (int)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)c->prop.multiProcessorCount * per_cu, blocks_useful));
+}
+
+// tallies of one point from its 16 fixed-point slot words
+static void slot_to_tally(const unsigned long long* v, uint64_t n_packets, ismc_tally* t) {
+  const double inv = 1.0 / 4294967296.0;
+  for (int k = 0; k < 5; ++k) {
+    t->sum[k] = (double)v[k] * inv;
+    t->sumsq[k] = (double)v[5 + k] * inv;
+    t->events[k] = v[10 + k];
+  }
+  t->n_aborted = v[15];
+  t->n_packets = n_packets;
+}
+
+template <typename R>
+static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
+                      const uint64_t* first, const uint32_t* streams, int32_t n_points,
+                      uint64_t seed, uint32_t flags, const ismc_tune* tune, ismc_tally* out,
+                      double* per_packet_host) {
+  int per_cu = 1, chunk = 1, rc;
+  if ((rc = launch_shape<R>(c, npk, n_points, tune, per_packet_host != nullptr, &per_cu, &chunk)))
+    return rc;
   std::vector<Range> ranges((size_t)n_points);
   uint64_t total = 0, n_chunks = 0;
-  for (int p = 0; p < n_points; ++p) {
-    if (npk[p] > (1ull << 32))
-      return set_err(c, ISMC_ERANGE, "point %s%lld: more than 2^32 packets in one call", "", p);
-    const uint64_t f0 = first ? first[p] : 0ull;
-    if (f0 + npk[p] < f0) return set_err(c, ISMC_ERANGE, "packet id range overflows%s");
-    if (n_chunks > 0xffffffffull) return set_err(c, ISMC_ERANGE, "too many chunks%s");
-    Range& r = ranges[(size_t)p];
-    r.first_pid = f0;
-    r.out_base = total;
-    r.n_packets = npk[p];
-    r.chunk0 = (uint32_t)n_chunks;
-    r.stream = streams ? streams[p] : (uint32_t)p;
-    n_chunks += (npk[p] + (uint64_t)chunk - 1) / (uint64_t)chunk;
-    total += npk[p];
-  }
+  for (int p = 0; p < n_points; ++p)
+    if ((rc = point_range(c, p, npk, first, streams, chunk, &n_chunks, &total, &ranges[(size_t)p])))
+      return rc;
   if (n_chunks > 0xffffffffull) return set_err(c, ISMC_ERANGE, "too many chunks%s");
   const size_t pbytes = sizeof(Point<R>) * (size_t)n_points;
   const size_t cbytes = sizeof(Range) * (size_t)n_points;
   const size_t sbytes = sizeof(unsigned long long) * kSlotWords * (size_t)n_points;
-  int rc;
   if ((rc = ensure_dev(c, c->points, pbytes))) return rc;
   if ((rc = ensure_dev(c, c->ranges, cbytes))) return rc;
   if ((rc = ensure_dev(c, c->slots, sbytes))) return rc;
@@ -1038,27 +1077,10 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
     if ((rc = ensure_dev(c, c->per_packet, sizeof(double) * 6 * std::max<uint64_t>(total, 1))))
       return rc;
   }
-
-  Args a{};
-  a.points = c->points.p;
-  a.ranges = (const Range*)c->ranges.p;
-  a.n_points = (uint32_t)n_points;
-  a.n_chunks = (uint32_t)n_chunks;
-  a.chunk_packets = (uint32_t)chunk;
-  a.seed_lo = (uint32_t)seed;
-  a.seed_hi = (uint32_t)(seed >> 32);
-  a.chunk_counter = (uint32_t*)c->counter.p;
-  a.slots = (unsigned long long*)c->slots.p;
-  a.per_packet = per_packet_host ? (double*)c->per_packet.p : nullptr;
-  a.idle_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->idle_lanes > 0) ? tune->idle_lanes
-                                                                                 : c->idle_lanes));
-  a.bound_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->bound_lanes > 0) ? tune->bound_lanes
-                                                                                   : c->bound_lanes));
-
-  const uint64_t waves_useful = std::max<uint64_t>(1, (total + 63) / 64);
-  const uint64_t blocks_useful = (waves_useful + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int grid = (int)std::max<uint64_t>(
-      1, std::min<uint64_t>((uint64_t)c->prop.multiProcessorCount * per_cu, blocks_useful));
+  const Args a = make_args(c, tune, c->points.p, c->ranges.p, c->slots.p, c->counter.p,
+                           per_packet_host ? (double*)c->per_packet.p : nullptr, n_points, n_chunks,
+                           chunk, seed);
+  const int grid = grid_for(c, per_cu, total);
   unsigned long long* hs = reinterpret_cast<unsigned long long*>(st + pbytes + cbytes);
   // the launch sequence: upload point tables and the range records from pinned
   // staging, clear the tally slots and queue counter with a kernel (no
@@ -1144,18 +1166,7 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   HIPCHECK(c, hipEventElapsedTime(&c->last_ms, c->e0, c->e1));
   c->last_grid = grid;
   c->last_chunks = (int64_t)n_chunks;
-  const double inv = 1.0 / 4294967296.0;
-  for (int p = 0; p < n_points; ++p) {
-    const unsigned long long* v = hs + (size_t)p * kSlotWords;
-    ismc_tally& t = out[p];
-    for (int k = 0; k < 5; ++k) {
-      t.sum[k] = (double)v[k] * inv;
-      t.sumsq[k] = (double)v[5 + k] * inv;
-      t.events[k] = v[10 + k];
-    }
-    t.n_aborted = v[15];
-    t.n_packets = npk[p];
-  }
+  for (int p = 0; p < n_points; ++p) slot_to_tally(hs + (size_t)p * kSlotWords, npk[p], &out[p]);
   return 0;
 }
 
@@ -1187,10 +1198,11 @@ int ismc_create(int device, ismc_ctx** out) {
   if (device < 0 || device >= n) return ISMC_EINVAL;
   ismc_ctx* c = new ismc_ctx();
   c->device = device;
+  c->err.reserve(512);  // set_err's messages then never reallocate
   if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&c->prop, device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
-    delete c;
+    ismc_destroy(c);
     return ISMC_EDEVICE;
   }
   *out = c;
@@ -1310,39 +1322,6 @@ int ismc_eval_physics(ismc_ctx* c, int32_t op, const ismc_config* cfg, const dou
   HIPCHECK(c, hipMemcpyAsync(out2, c->eval_out.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return 0;
-}
-
-int ismc_copy_probe(ismc_ctx* c, int64_t bytes, float* kernel_ms) {
-  if (!c || bytes <= 0 || (bytes % 16) != 0) return set_err(c, ISMC_EINVAL, "copy probe needs a positive multiple of 16 bytes%s");
-  HIPCHECK(c, hipSetDevice(c->device));
-  void* a = nullptr;
-  void* b = nullptr;
-  HIPCHECK(c, hipMalloc(&a, (size_t)bytes));
-  hipError_t e = hipMalloc(&b, (size_t)bytes);
-  if (e != hipSuccess) {
-    (void)hipFree(a);
-    return set_err(c, ISMC_ENOMEM, "hipMalloc(%s) for the copy probe", hipGetErrorString(e));
-  }
-  const size_t n = (size_t)bytes / 16;
-  const int grid = c->prop.multiProcessorCount * 8;
-  // one exit path, so that both buffers are freed whatever fails
-  auto timed_copy = [&]() -> int {
-    copy_probe_kernel<<<grid, 256, 0, c->stream>>>((const uint4*)b, (uint4*)a, n);  // warm
-    HIPCHECK(c, hipGetLastError());
-    HIPCHECK(c, hipEventRecord(c->e0, c->stream));
-    copy_probe_kernel<<<grid, 256, 0, c->stream>>>((const uint4*)a, (uint4*)b, n);
-    HIPCHECK(c, hipGetLastError());
-    HIPCHECK(c, hipEventRecord(c->e1, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    float ms = 0.f;
-    HIPCHECK(c, hipEventElapsedTime(&ms, c->e0, c->e1));
-    if (kernel_ms) *kernel_ms = ms;
-    return 0;
-  };
-  const int rc = timed_copy();
-  (void)hipFree(a);
-  (void)hipFree(b);
-  return rc;
 }
 
 }  // extern "C"

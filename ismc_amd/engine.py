@@ -217,13 +217,6 @@ class Engine:
         self._L.ismc_graph_stats(self._ctx, ctypes.byref(cap), ctypes.byref(lau))
         return {"captures": cap.value, "launches": lau.value}
 
-    def copy_probe(self, nbytes: int) -> float:
-        """Device ms of a plain device-to-device copy of nbytes (counter calibration)."""
-        ms = ctypes.c_float()
-        _lib.check(self._ctx, self._L.ismc_copy_probe(self._ctx, int(nbytes), ctypes.byref(ms)),
-                   "ismc_copy_probe")
-        return ms.value
-
     def eval_physics(self, op: int, inputs, cfg: Optional[SampleConfig] = None, fp64=False):
         x = np.ascontiguousarray(np.asarray(inputs, dtype=np.float64).reshape(-1, 4))
         out = np.zeros((x.shape[0], 2), dtype=np.float64)

@@ -212,6 +212,7 @@ class BatchCost:
                                                         use_graph=self.use_graph, **self.launch)
         self.launches += 1
         self.packets += self.n_packets * k
+        self.kernel_ms += self.engine.last_launch()["kernel_ms"]
         mean = sums / npk[:, None].astype(np.float64)
         m = self._meas
         c = np.zeros(k)

@@ -72,15 +72,23 @@ def _hypot(x, y):
 
 
 def _minfit(a, tol):
-    """Singular value decomposition by Golub and Reinsch's method in the form
-    Brent gives for PRAXIS (Algorithms for Minimization without Derivatives,
-    ch. 7, procedure minfit): Householder reduction of the n x n matrix `a`
-    (list of rows) to bidiagonal form, accumulation of the right-hand
-    transformations, and implicitly shifted QR on the bidiagonal.  On return
-    `a` holds the right singular vectors as columns; the singular values (not
-    sorted, all >= 0) are returned.  Rotations are applied in the same order
-    and with the same sign conventions as that procedure, so its output is
-    reproduced to the last bit."""
+    """Singular value decomposition by Golub and Reinsch's method.
+
+    Sources: G. H. Golub and C. Reinsch, "Singular value decomposition and
+    least squares solutions", Numer. Math. 14 (1970) 403-420 (the procedure
+    that EISPACK publishes as `minfit`), in the form R. P. Brent gives for
+    PRAXIS (Algorithms for Minimization without Derivatives, Prentice-Hall
+    1973, ch. 7, procedure minfit).  The arithmetic-order target is the
+    reference's C++ minfit (CUDA/praxis.cpp:130-556), whose iterates
+    tests/test_praxis.py compares bit for bit.
+
+    Householder reduction of the n x n matrix `a` (list of rows) to
+    bidiagonal form, accumulation of the right-hand transformations, and
+    implicitly shifted QR on the bidiagonal.  On return `a` holds the right
+    singular vectors as columns; the singular values (not sorted, all >= 0)
+    are returned.  Rotations are applied in the same order and with the same
+    sign conventions as that procedure, so its output is reproduced to the
+    last bit."""
     n = len(a)
     if n == 1:
         q = [a[0][0]]

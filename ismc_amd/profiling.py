@@ -18,6 +18,8 @@ activity counters per packet.
 `--issue-rates` instead measures the device's VALU issue cost per wave64
 instruction for single instruction kinds (ismc_issue_rate) and prints them
 as JSON: the per-cycle figures behind the transport kernel's roofline.
+The probes (ismc_issue_rate, ismc_copy_probe) live in libismc_diag.so
+(include/ismc_diag.h), beside the product library and not in it.
 
     python -m ismc_amd.profiling --issue-rates
 """
@@ -113,21 +115,59 @@ def summarize(packets, copy_bytes, trace=None, fetch=None, write=None, pmc=(), k
     return out
 
 
+DIAG_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libismc_diag.so")
+_diag_lib = None
+
+
+def diag():
+    """Load libismc_diag.so (the measurement probes) and declare its
+    signatures; no GPU is needed to load it."""
+    global _diag_lib
+    if _diag_lib is None:
+        import ctypes
+        if not os.path.exists(DIAG_PATH):
+            raise RuntimeError("libismc_diag.so is not built (%s); run `python -m ismc_amd.build`" % DIAG_PATH)
+        L = ctypes.CDLL(DIAG_PATH)
+        P = ctypes.POINTER
+        L.ismc_issue_rate_ops.argtypes = []
+        L.ismc_issue_rate_ops.restype = ctypes.c_int32
+        L.ismc_issue_rate_name.argtypes = [ctypes.c_int32]
+        L.ismc_issue_rate_name.restype = ctypes.c_char_p
+        L.ismc_issue_rate.argtypes = [ctypes.c_int, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_double)]
+        L.ismc_issue_rate.restype = ctypes.c_int
+        L.ismc_copy_probe.argtypes = [ctypes.c_int, ctypes.c_int64, P(ctypes.c_float)]
+        L.ismc_copy_probe.restype = ctypes.c_int
+        L.ismc_diag_last_error.argtypes = []
+        L.ismc_diag_last_error.restype = ctypes.c_char_p
+        _diag_lib = L
+    return _diag_lib
+
+
+def copy_probe(device, nbytes):
+    """Device ms of a plain device-to-device copy of nbytes (the byte
+    counters' calibration: copy_probe_kernel in a profile run)."""
+    import ctypes
+    L = diag()
+    ms = ctypes.c_float()
+    if L.ismc_copy_probe(int(device), int(nbytes), ctypes.byref(ms)) != 0:
+        raise RuntimeError("ismc_copy_probe: %s" % L.ismc_diag_last_error().decode())
+    return ms.value
+
+
 def issue_rates(device=0):
     """Wave64 instructions per second, shader clock and cycles per
     instruction for each instruction kind of ismc_issue_rate."""
     import ctypes
 
-    from . import _lib
     from .engine import Engine
-    L = _lib.load()
+    L = diag()
     n_cu = Engine(device).device_info()["n_cu"]
     out = []
     for k in range(L.ismc_issue_rate_ops()):
         rate, ghz = ctypes.c_double(), ctypes.c_double()
         rc = L.ismc_issue_rate(int(device), k, ctypes.byref(rate), ctypes.byref(ghz))
         if rc != 0:
-            raise RuntimeError("ismc_issue_rate(%d): %s" % (k, L.ismc_issue_rate_last_error().decode()))
+            raise RuntimeError("ismc_issue_rate(%d): %s" % (k, L.ismc_diag_last_error().decode()))
         simds = 4 * n_cu
         out.append({"op": L.ismc_issue_rate_name(k).decode(), "wave_insts_per_s": rate.value,
                     "shader_ghz": ghz.value,

@@ -15,23 +15,40 @@ from ismc_amd.workloads import slab_between_slides, three_layer_sample
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    names = []
-    for h in os.listdir(os.path.join(ROOT, "include")):
-        if h.endswith(".h"):
-            txt = open(os.path.join(ROOT, "include", h)).read()
-            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-            names += re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", txt, flags=re.M)
-    return sorted(set(names))
+def declared_functions(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_every_header_has_its_library():
+    assert sorted(h for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h")) == \
+        ["ismc.h", "ismc_diag.h"]
 
 
 def test_library_exports_every_declared_function():
     L = _lib.load()
-    names = declared_functions()
+    names = declared_functions("ismc.h")
     assert "ismc_simulate" in names and "ismc_simulate_batch" in names
     for n in names:
         assert hasattr(L, n), "missing export " + n
     assert set(_lib.EXPORTS) <= set(names)
+
+
+def test_product_library_exports_no_probe():
+    L = _lib.load()
+    for n in declared_functions("ismc_diag.h"):
+        assert not hasattr(L, n), "probe %s is exported by the product library" % n
+
+
+def test_diag_library_exports_every_declared_function():
+    from ismc_amd import profiling
+    D = profiling.diag()
+    names = declared_functions("ismc_diag.h")
+    assert "ismc_copy_probe" in names and "ismc_issue_rate" in names
+    for n in names:
+        assert hasattr(D, n), "missing export " + n
+    assert D.ismc_issue_rate_ops() > 0
 
 
 def test_struct_sizes_match_header_layout():

@@ -273,9 +273,25 @@ def test_errors_are_raised_not_exited(engine):
     with pytest.raises(ValueError):
         engine.simulate(bad, 1000)
     with pytest.raises(ValueError):
-        engine.simulate(slab_between_slides(), (1 << 32) + 1)
+        engine.simulate(slab_between_slides(), (1 << 31) + 1)
     ok = engine.simulate(slab_between_slides(), 1000)
     assert ok.n_packets == 1000
+
+
+def test_largest_point_keeps_its_tallies(engine):
+    """A point of exactly 2^31 packets (the per-call limit) of an
+    index-matched, almost purely absorbing layer: every packet leaves its
+    whole weight in A, the worst case for the 64-bit fixed-point sums (2^63
+    of their 2^64 range; the 2^32 packets the limit once allowed wrapped
+    this sum to 0).  A and A^2 must still average 1."""
+    cfg = SampleConfig(layers=[Layer(n=1.0, g=0.0, t=1.0, mu_a=1000.0, mu_s=1e-9)],
+                       R=Sphere(False, angle=0.0), T=Sphere(False, angle=0.0))
+    n = 1 << 31
+    r = engine.simulate(cfg, n, seed=5)
+    assert r.n_packets == n and r.n_aborted == 0
+    assert abs(r.mean["A"] - 1.0) < 1e-6, r.mean
+    assert abs(r.sumsq["A"] / n - 1.0) < 1e-6, r.sumsq
+    assert sum(r.mean.values()) == pytest.approx(1.0, abs=1e-6)
 
 
 def test_reference_style_ctypes_binding(engine):
@@ -324,15 +340,17 @@ def test_reference_style_ctypes_binding(engine):
 
 def test_error_returns_and_zero_packets(engine):
     """Host-side failures come back as status codes with a message, before
-    any launch (the reference exits the process): more than 2^32 packets in
-    one point and a packet-id range that wraps are ERANGE, an invalid
+    any launch (the reference exits the process): more than 2^31 packets in
+    one point (2^32 of them could wrap a 64-bit fixed-point tally) and a
+    packet-id range that wraps are ERANGE, an invalid
     configuration or a null array EINVAL; a call for 0 packets returns
     empty tallies.  The context stays usable after each."""
     import ctypes
     from ismc_amd import _lib
     cfg = single_layer(1.0, 10.0)
-    with pytest.raises(ValueError, match="2\\^32"):
-        engine.simulate(cfg, (1 << 32) + 1)
+    for n in (1 << 32, (1 << 31) + 1):
+        with pytest.raises(ValueError, match="2\\^31"):
+            engine.simulate(cfg, n)
     with pytest.raises(ValueError, match="overflow"):
         engine.simulate(cfg, 1000, first_packet_id=(1 << 64) - 10)
     bad = SampleConfig(layers=[Layer(n=1.4, g=1.0, t=0.1, mu_a=1.0, mu_s=10.0)], R=cfg.R, T=cfg.T)

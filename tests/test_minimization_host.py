@@ -39,6 +39,9 @@ class AnalyticEngine:
         npk = np.full(len(cfg), int(n), dtype=np.uint64)
         return tal * n, tal * n, npk
 
+    def last_launch(self):
+        return {"kernel_ms": 0.25, "grid": 1, "block": 256, "chunks": 1}
+
     def simulate_batch(self, cfgs, n, seed=0, streams=None, use_graph=False, **kw):
         self.calls.append(len(cfgs))
         out = []
@@ -52,6 +55,15 @@ class AnalyticEngine:
 def problem(measured, separate=False):
     S = Sphere(True, 0.98, 0.0039, 0.0039, 0.001)
     return M.Problem(n=1.4, g=0.8, t=0.1, R=S, T=S, measured=measured, separate=separate)
+
+
+def test_batch_cost_counts_kernel_time_on_the_array_path():
+    eng = AnalyticEngine()
+    probs = [problem(M.Measured(R_d=0.3, T_d=0.4)) for _ in range(3)]
+    cost = M.BatchCost(eng, probs, 1000)
+    cost([(1.0, 10.0), (2.0, 5.0), (0.5, 20.0)], [0, 1, 2])
+    cost([(1.0, 11.0)], [1])
+    assert cost.launches == 2 and cost.kernel_ms == pytest.approx(0.5)
 
 
 def test_cost_formula():
