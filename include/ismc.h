@@ -139,6 +139,27 @@ int ismc_simulate_batch(ismc_ctx* ctx, const ismc_config* cfgs, const uint64_t* 
                         int32_t n_points, uint64_t seed, uint32_t flags, const ismc_tune* tune,
                         ismc_tally* out);
 
+/* No counterpart in the reference: for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  ismc_simulate_batch without any allocation: the
+ * launch's device buffers (point tables, queue records, tally slots,
+ * queue counter) live in a static device array of
+ * ISMC_STATIC_SCRATCH_BYTES that every GPU the library runs on holds until
+ * the process ends, and the host side stages through a pinned buffer that
+ * ismc_create allocated.  At most ISMC_STATIC_MAX_POINTS configurations
+ * (FP32 or FP64) per call, else ISMC_ERANGE before any device work; the
+ * other limits and the tallies are those of ismc_simulate_batch, bit for
+ * bit.  tune->use_graph is ignored (the call runs eagerly).  All calls on
+ * one GPU share its array: calls that overlap on one GPU (from two
+ * contexts or threads) race.  ISMC_EDEVICE if the array's address could
+ * not be read at ismc_create. */
+#define ISMC_STATIC_MAX_POINTS 1024
+#define ISMC_STATIC_SCRATCH_BYTES 2982144
+int ismc_simulate_batch_static(ismc_ctx* ctx, const ismc_config* cfgs, const uint64_t* n_packets,
+                               const uint64_t* first_packet_id, const uint32_t* streams,
+                               int32_t n_points, uint64_t seed, uint32_t flags,
+                               const ismc_tune* tune, ismc_tally* out);
+
 /* Per-packet records {A, R_d, R_s, T_d, T_u, events} (n_packets x 6
  * doubles) for one configuration: a debugging and parity aid. */
 int ismc_simulate_packets(ismc_ctx* ctx, const ismc_config* cfg, uint64_t n_packets,

@@ -19,7 +19,8 @@ _STATUS = {EINVAL: "invalid argument", EDEVICE: "HIP error", ENOMEM: "out of mem
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libismc.so")
 
 EXPORTS = ("ismc_create", "ismc_destroy", "ismc_last_error", "ismc_device_info_get",
-           "ismc_simulate", "ismc_simulate_batch", "ismc_simulate_packets", "ismc_last_launch",
+           "ismc_simulate", "ismc_simulate_batch", "ismc_simulate_batch_static",
+           "ismc_simulate_packets", "ismc_last_launch",
            "ismc_eval_physics", "ismc_validate_config", "ismc_graph_stats",
            "ismc_table_uncertainty", "ismc_table_last_error", "ismc_grid_argmin")
 
@@ -91,6 +92,8 @@ def load():
                                           P(ctypes.c_uint32), ctypes.c_int32, ctypes.c_uint64,
                                           ctypes.c_uint32, P(Tune), P(Tally)]
         L.ismc_simulate_batch.restype = ctypes.c_int
+        L.ismc_simulate_batch_static.argtypes = L.ismc_simulate_batch.argtypes
+        L.ismc_simulate_batch_static.restype = ctypes.c_int
         L.ismc_simulate_packets.argtypes = [vp, P(Config), ctypes.c_uint64, ctypes.c_uint64,
                                             ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                             vp, P(Tally)]

@@ -60,6 +60,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -716,6 +717,10 @@ struct ismc_ctx {
   DevBuf points, ranges, slots, counter, per_packet, eval_in, eval_out;
   void* h_stage = nullptr;  // pinned staging
   size_t h_stage_cap = 0;
+  // no-allocation launches: this device's address of the static scratch
+  // array, and a fixed pinned staging buffer allocated with the context
+  void* scratch = nullptr;
+  void* h_fixed = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   float last_ms = 0.f;
   int32_t last_grid = 0;
@@ -1170,10 +1175,121 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// No-allocation launches (ismc_simulate_batch_static).  Every device buffer
+// of such a launch (point tables, queue range records, tally slots, queue
+// counter) lives in one file-scope device array, `scratch`, which each GPU
+// the library runs on holds until the process ends.  In host code the
+// array's name only stands for the symbol: the host reaches the memory
+// through the address hipGetSymbolAddress returns with the device current
+// (kept per device in g_scratch_addr, looked up once per device when a
+// context is created), and the kernels take that address as an argument.
+// All calls on one GPU share its array and the last call's contents stay
+// there: the slots and the counter are cleared by a kernel at every launch,
+// and calls that overlap on one GPU race.
+constexpr int32_t kStaticMaxPoints = ISMC_STATIC_MAX_POINTS;
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kStaticPointBytes = align256(sizeof(Point<double>) * kStaticMaxPoints);
+constexpr size_t kStaticRangeBytes = align256(sizeof(Range) * kStaticMaxPoints);
+constexpr size_t kStaticSlotBytes = align256(sizeof(unsigned long long) * kSlotWords * kStaticMaxPoints);
+constexpr size_t kStaticBytes = kStaticPointBytes + kStaticRangeBytes + kStaticSlotBytes + 256;
+static_assert(sizeof(Point<float>) <= sizeof(Point<double>), "FP32 tables fit the FP64 section");
+static_assert(kStaticBytes == ISMC_STATIC_SCRATCH_BYTES, "ismc.h states the scratch size");
+
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kMaxDevices];
+
+// this device's address of `scratch`; the device must be current
+static hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return hipSuccess;
+}
+
+constexpr size_t kFixedStageBytes = 256 * 1024;  // pinned, allocated with the context
+
+template <typename R>
+static int run_points_static(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
+                             const uint64_t* first, const uint32_t* streams, int32_t n_points,
+                             uint64_t seed, uint32_t flags, const ismc_tune* tune, ismc_tally* out) {
+  if (!c->scratch || !c->h_fixed)
+    return set_err(c, ISMC_EDEVICE, "no static scratch address for this device%s");
+  if (n_points > kStaticMaxPoints)
+    return set_err(c, ISMC_ERANGE, "at most %s%lld configurations per no-allocation call", "",
+                   kStaticMaxPoints);
+  int per_cu = 1, chunk = 1, rc;
+  if ((rc = launch_shape<R>(c, npk, n_points, tune, false, &per_cu, &chunk))) return rc;
+  // every range is checked before any device work
+  uint64_t total = 0, n_chunks = 0;
+  for (int p = 0; p < n_points; ++p) {
+    Range r;
+    if ((rc = point_range(c, p, npk, first, streams, chunk, &n_chunks, &total, &r))) return rc;
+  }
+  if (n_chunks > 0xffffffffull) return set_err(c, ISMC_ERANGE, "too many chunks%s");
+  char* base = (char*)c->scratch;
+  Point<R>* d_points = reinterpret_cast<Point<R>*>(base);
+  Range* d_ranges = reinterpret_cast<Range*>(base + kStaticPointBytes);
+  unsigned long long* d_slots = reinterpret_cast<unsigned long long*>(base + kStaticPointBytes + kStaticRangeBytes);
+  uint32_t* d_counter = reinterpret_cast<uint32_t*>(base + kStaticPointBytes + kStaticRangeBytes + kStaticSlotBytes);
+  // point tables and range records through the fixed pinned buffer, a
+  // batch at a time (each batch's copy finishes before the buffer is reused)
+  char* st = (char*)c->h_fixed;
+  const int per_batch = (int)(kFixedStageBytes / (sizeof(Point<R>) + sizeof(Range)));
+  uint64_t t_run = 0, ch_run = 0;
+  for (int p0 = 0; p0 < n_points; p0 += per_batch) {
+    const int nb = std::min(per_batch, n_points - p0);
+    Point<R>* hp = reinterpret_cast<Point<R>*>(st);
+    Range* hr = reinterpret_cast<Range*>(st + sizeof(Point<R>) * (size_t)nb);
+    for (int i = 0; i < nb; ++i) {
+      compile_point<R>(&cfgs[p0 + i], flags, &hp[i]);
+      if ((rc = point_range(c, p0 + i, npk, first, streams, chunk, &ch_run, &t_run, &hr[i]))) return rc;
+    }
+    HIPCHECK(c, hipMemcpyAsync(d_points + p0, hp, sizeof(Point<R>) * (size_t)nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_ranges + p0, hr, sizeof(Range) * (size_t)nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  const size_t nslot = (size_t)kSlotWords * n_points;
+  const Args a = make_args(c, tune, d_points, d_ranges, d_slots, d_counter, nullptr, n_points, n_chunks,
+                           chunk, seed);
+  const int grid = grid_for(c, per_cu, total);
+  zero_u64_kernel<<<(unsigned)((nslot + 255) / 256), 256, 0, c->stream>>>(d_slots, nslot, d_counter);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipEventRecord(c->e0, c->stream));
+  transport_kernel<R, false><<<grid, kBlock, 0, c->stream>>>(a);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipEventRecord(c->e1, c->stream));
+  // tallies back through the same buffer
+  const int slot_batch = (int)(kFixedStageBytes / (sizeof(unsigned long long) * kSlotWords));
+  const unsigned long long* hs = reinterpret_cast<const unsigned long long*>(st);
+  for (int p0 = 0; p0 < n_points; p0 += slot_batch) {
+    const int nb = std::min(slot_batch, n_points - p0);
+    HIPCHECK(c, hipMemcpyAsync(st, d_slots + (size_t)p0 * kSlotWords,
+                               sizeof(unsigned long long) * kSlotWords * (size_t)nb, hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < nb; ++i) slot_to_tally(hs + (size_t)i * kSlotWords, npk[p0 + i], &out[p0 + i]);
+  }
+  HIPCHECK(c, hipEventElapsedTime(&c->last_ms, c->e0, c->e1));
+  c->last_grid = grid;
+  c->last_chunks = (int64_t)n_chunks;
+  return 0;
+}
+
 static int run_dispatch(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
                         const uint64_t* first, const uint32_t* streams, int32_t n_points,
                         uint64_t seed, uint32_t flags, const ismc_tune* tune, ismc_tally* out,
-                        double* per_packet_host) {
+                        double* per_packet_host, bool no_alloc = false) {
   if (!c) return ISMC_EINVAL;
   if (!cfgs || !npk || !out || n_points < 1) return set_err(c, ISMC_EINVAL, "null argument%s");
   for (int p = 0; p < n_points; ++p) {
@@ -1181,6 +1297,11 @@ static int run_dispatch(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* np
     if (rc) return rc;
   }
   HIPCHECK(c, hipSetDevice(c->device));
+  if (no_alloc) {
+    if (flags & ISMC_FLAG_FP64)
+      return run_points_static<double>(c, cfgs, npk, first, streams, n_points, seed, flags, tune, out);
+    return run_points_static<float>(c, cfgs, npk, first, streams, n_points, seed, flags, tune, out);
+  }
   if (flags & ISMC_FLAG_FP64)
     return run_points<double>(c, cfgs, npk, first, streams, n_points, seed, flags, tune, out,
                               per_packet_host);
@@ -1205,6 +1326,12 @@ int ismc_create(int device, ismc_ctx** out) {
     ismc_destroy(c);
     return ISMC_EDEVICE;
   }
+  // set-up of the no-allocation entry point: this device's address of the
+  // static scratch array (once per device) and the fixed pinned staging
+  // buffer; without them that entry point reports ISMC_EDEVICE and every
+  // other one works as before
+  if (scratch_address(device, &c->scratch) != hipSuccess) c->scratch = nullptr;
+  if (hipHostMalloc(&c->h_fixed, kFixedStageBytes, hipHostMallocDefault) != hipSuccess) c->h_fixed = nullptr;
   *out = c;
   return 0;
 }
@@ -1219,6 +1346,7 @@ void ismc_destroy(ismc_ctx* c) {
   for (DevBuf* b : bufs)
     if (b->p) e = hipFree(b->p);
   if (c->h_stage) e = hipHostFree(c->h_stage);
+  if (c->h_fixed) e = hipHostFree(c->h_fixed);
   if (c->e0) e = hipEventDestroy(c->e0);
   if (c->e1) e = hipEventDestroy(c->e1);
   if (c->stream) e = hipStreamDestroy(c->stream);
@@ -1259,6 +1387,14 @@ int ismc_simulate_batch(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* n_
                         ismc_tally* out) {
   return run_dispatch(c, cfgs, n_packets, first_packet_id, streams, n_points, seed, flags, tune,
                       out, nullptr);
+}
+
+int ismc_simulate_batch_static(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* n_packets,
+                               const uint64_t* first_packet_id, const uint32_t* streams,
+                               int32_t n_points, uint64_t seed, uint32_t flags, const ismc_tune* tune,
+                               ismc_tally* out) {
+  return run_dispatch(c, cfgs, n_packets, first_packet_id, streams, n_points, seed, flags, tune, out,
+                      nullptr, true);
 }
 
 int ismc_simulate_packets(ismc_ctx* c, const ismc_config* cfg, uint64_t n_packets, uint64_t seed,
