@@ -202,7 +202,14 @@ class Engine:
             st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if st is not None else None, npts, int(seed),
             self._flags(fp64, False), ctypes.byref(tune), outs.ctypes.data_as(ctypes.POINTER(_lib.Tally)))
         _lib.check(self._ctx, rc, "ismc_simulate_batch")
+        # event counts of the launch (interactions, boundary hits, entrances,
+        # sphere decisions, re-injections), summed over its points
+        self._last_events = dict(zip(EVENTS, (int(x) for x in outs["events"].sum(axis=0))))
         return outs["sum"], outs["sumsq"], outs["n_packets"]
+
+    def last_events(self):
+        """Event counts of the last simulate_batch_array launch, summed over its points."""
+        return dict(getattr(self, "_last_events", dict.fromkeys(EVENTS, 0)))
 
     @property
     def tuner(self):

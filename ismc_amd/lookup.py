@@ -212,28 +212,47 @@ def run_lookup(ini, samples=None, seed=1234, engine: Optional[Engine] = None, rn
 
 def adaptive_sweep(cfgs: List[SampleConfig], rse=0.005, pilot=10_000, n_min=10_000,
                    n_max=1_000_000, seed=1234, engine: Optional[Engine] = None,
-                   chunk_packets=0, blocks_per_cu=0, autotune=False):
+                   chunk_packets=0, blocks_per_cu=0, autotune=False, topups=3, margin=1.05):
     """Uneven packet counts per configuration: a pilot of `pilot` packets
     estimates total reflectance R = R_d + R_s and its per-packet variance
     (a packet leaves its weight in exactly one exit tally, so
     Var(R_d + R_s) = E[R_d^2] + E[R_s^2] - R^2); each point then gets
     N = clamp(ceil(Var / (R rse)^2), n_min, n_max) packets in total, the
     remaining N - pilot in one launch on packet ids after the pilot's.
-    The configurations go to the library once, as an array, and the tallies
-    come back as arrays (no per-point objects until the result list)."""
+    The pilot's estimate is noisy, so a point whose relative standard error
+    of R is still above `rse` after that launch, and that has not reached
+    n_max, is topped up: up to `topups` more launches, each running every
+    such point's shortfall, ceil(margin Var / (R rse)^2) - N (at most
+    n_max - N), on the packet ids after its last ones, re-estimated from all
+    its packets so far.  The configurations go to the library once, as an
+    array, and the tallies come back as arrays (no per-point objects until
+    the result list)."""
     eng = engine or default_engine()
     npts = len(cfgs)
     streams = np.arange(npts, dtype=np.uint32)
     arr = np.frombuffer(b"".join(bytes(c.to_c()) for c in cfgs), dtype=CONFIG_DTYPE).copy()
+
+    events = {}
 
     def launch(n_packets, first=None):
         kw = {"chunk_packets": chunk_packets, "blocks_per_cu": blocks_per_cu}
         if autotune:
             kw = eng.tuner.params(cfgs, n_packets, seed=seed)
         out = eng.simulate_batch_array(arr, n_packets, seed=seed, streams=streams, first_packet_ids=first, **kw)
+        if hasattr(eng, "last_events"):
+            for k, v in eng.last_events().items():
+                events[k] = events.get(k, 0) + v
         return out, eng.last_launch()["kernel_ms"]
 
     iRd, iRs = TALLIES.index("R_d"), TALLIES.index("R_s")
+
+    def rse_of(S, Q, tot):
+        Rm = S[:, iRd] / tot + S[:, iRs] / tot
+        varR = (Q[:, iRd] + Q[:, iRs]) / tot - Rm * Rm
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = np.where(Rm > 0, np.sqrt(np.maximum(varR, 0) / tot) / Rm, float("inf"))
+        return r, Rm, varR
+
     (s1, q1, n1), pilot_ms = launch(pilot)
     n1 = n1.astype(np.float64)
     # the plan, in the scalar form's order of operations (same packet counts)
@@ -247,17 +266,34 @@ def adaptive_sweep(cfgs: List[SampleConfig], rse=0.005, pilot=10_000, n_min=10_0
         (s2, q2, n2), main_ms = launch(extra.astype(np.uint64), first=np.full(npts, pilot, dtype=np.uint64))
     else:
         s2, q2, n2, main_ms = np.zeros_like(s1), np.zeros_like(q1), np.zeros(npts, dtype=np.uint64), 0.0
-    tot = (n1 + n2.astype(np.float64))[:, None]
     S, Q = s1 + s2, q1 + q2
+    done = n1.astype(np.int64) + n2.astype(np.int64)
+    topup_ms, topup_packets, topup_launches = 0.0, 0, 0
+    for _ in range(int(topups)):
+        r, Rm, varR = rse_of(S, Q, done.astype(np.float64))
+        short = (r > rse * (1 + 1e-9)) & (done < n_max) & (Rm > 0)
+        if not short.any():
+            break
+        with np.errstate(divide="ignore", invalid="ignore"):
+            want = np.ceil(margin * np.maximum(varR, 0.0) / (Rm * rse) ** 2)
+        add = np.where(short, np.clip(want - done, 1, n_max - done), 0).astype(np.int64)
+        (s3, q3, n3), ms = launch(add.astype(np.uint64), first=done.astype(np.uint64))
+        S, Q = S + s3, Q + q3
+        done = done + n3.astype(np.int64)
+        topup_ms += ms
+        topup_packets += int(add.sum())
+        topup_launches += 1
+    tot = done.astype(np.float64)[:, None]
     mean = S / tot
     se = np.sqrt(np.maximum(Q / tot - mean ** 2, 0.0) / np.maximum(tot - 1, 1))
-    Rm = mean[:, iRd] + mean[:, iRs]
-    varR = (Q[:, iRd] + Q[:, iRs]) / tot[:, 0] - Rm * Rm
-    with np.errstate(divide="ignore", invalid="ignore"):
-        rse_R = np.where(Rm > 0, np.sqrt(np.maximum(varR, 0) / tot[:, 0]) / Rm, float("inf"))
-    mean_l, se_l, tot_l, rse_l = mean.tolist(), se.tolist(), tot[:, 0].astype(np.int64).tolist(), rse_R.tolist()
+    rse_R, _, _ = rse_of(S, Q, tot[:, 0])
+    mean_l, se_l, tot_l, rse_l = mean.tolist(), se.tolist(), done.tolist(), rse_R.tolist()
     out = [{"n_packets": tot_l[k], "mean": dict(zip(TALLIES, mean_l[k])), "se": dict(zip(TALLIES, se_l[k])),
             "rse_R": rse_l[k]} for k in range(npts)]
-    info = {"pilot_kernel_ms": pilot_ms, "main_kernel_ms": main_ms,
-            "total_packets": int(pilot * npts + extra.sum()), "planned": need}
+    capped = done >= n_max
+    info = {"pilot_kernel_ms": pilot_ms, "main_kernel_ms": main_ms, "topup_kernel_ms": topup_ms,
+            "topup_launches": topup_launches, "topup_packets": topup_packets,
+            "total_packets": int(done.sum()), "planned": need, "capped_points": int(capped.sum()),
+            "max_rse_R_uncapped": float(rse_R[~capped].max()) if (~capped).any() else 0.0,
+            "events": events}
     return out, info

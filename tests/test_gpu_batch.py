@@ -71,9 +71,10 @@ def test_adaptive_sweep_reaches_target_rse(engine):
     out, info = lookup.adaptive_sweep(cfgs, rse=0.005, pilot=10000, n_min=10000, n_max=1_000_000,
                                       seed=11, engine=engine)
     for o in out:
-        assert o["rse_R"] <= 0.0056 or o["n_packets"] == 1_000_000
+        assert o["rse_R"] <= 0.005 or o["n_packets"] == 1_000_000
         assert 10000 <= o["n_packets"] <= 1_000_000
     assert info["total_packets"] == sum(o["n_packets"] for o in out)
+    assert info["max_rse_R_uncapped"] <= 0.005
 
 
 def test_adaptive_sweep_matches_per_point_plan(engine):
@@ -84,9 +85,10 @@ def test_adaptive_sweep_matches_per_point_plan(engine):
     import math
     mua, mus, cfgs = lookup_grid(3, 4)
     pilot, n_min, n_max, rse = 5000, 8000, 400_000, 0.004
-    out, info = lookup.adaptive_sweep(cfgs, rse=rse, pilot=pilot, n_min=n_min, n_max=n_max, seed=21, engine=engine)
+    out, info = lookup.adaptive_sweep(cfgs, rse=rse, pilot=pilot, n_min=n_min, n_max=n_max, seed=21, engine=engine,
+                                      topups=0)
     tuned, _ = lookup.adaptive_sweep(cfgs, rse=rse, pilot=pilot, n_min=n_min, n_max=n_max, seed=21, engine=engine,
-                                     autotune=True)
+                                     autotune=True, topups=0)
     streams = np.arange(len(cfgs), dtype=np.uint32)
     p = engine.simulate_batch(cfgs, pilot, seed=21, streams=streams)
     need = []
@@ -108,6 +110,22 @@ def test_adaptive_sweep_matches_per_point_plan(engine):
     assert info["total_packets"] == sum(need)
 
 
+def test_adaptive_sweep_top_up_runs_the_next_packet_ids(engine):
+    """Top-up launches run each short point's next packet ids: the sweep's
+    tallies equal one run of the point's final packet count on its stream,
+    bit for bit, and every point below the cap meets the target."""
+    mua, mus, cfgs = lookup_grid(3, 4)
+    out, info = lookup.adaptive_sweep(cfgs, rse=0.004, pilot=2000, n_min=2000, n_max=400_000, seed=23,
+                                      engine=engine)
+    assert info["topup_launches"] >= 1 and info["topup_packets"] > 0
+    assert info["max_rse_R_uncapped"] <= 0.004
+    for k, o in enumerate(out):
+        assert o["rse_R"] <= 0.004 or o["n_packets"] == 400_000
+        one = engine.simulate(cfgs[k], o["n_packets"], seed=23, stream=k)
+        for t in one.mean:
+            assert one.sum[t] / one.n_packets == o["mean"][t], (k, t)
+
+
 def test_full_lookup_grid_sweep(engine):
     # The 64 x 64 lookup grid at its full size (mu_a 0.01-10, mu_s 1-1000,
     # adaptive 1e4-1e6 packets per point, 0.5 % RSE target), with
@@ -119,7 +137,8 @@ def test_full_lookup_grid_sweep(engine):
                                       seed=4321, engine=engine, autotune=True)
     n = np.array([o["n_packets"] for o in out])
     rse = np.array([o["rse_R"] for o in out])
-    assert np.all((rse <= 0.0056) | (n == 1_000_000)) and np.all((n >= 10000) & (n <= 1_000_000))
+    assert np.all((rse <= 0.005) | (n == 1_000_000)) and np.all((n >= 10000) & (n <= 1_000_000))
+    assert info["max_rse_R_uncapped"] <= 0.005 and info["capped_points"] == int((n == 1_000_000).sum())
     assert info["total_packets"] == n.sum() > 4096 * 10000
     for k in (0, 63, 2080, 4032, 4095):
         one = engine.simulate(cfgs[k], int(n[k]), seed=4321, stream=k)

@@ -84,6 +84,19 @@ class BernoulliEngine:
         return {"kernel_ms": 1.5}
 
 
+class BiasedPilotEngine(BernoulliEngine):
+    """The same stand-in, except that the first launch (the pilot) reports
+    R_d 1.6 times too high: the plan then asks too few packets and the
+    statistics of pilot + main miss the target."""
+
+    def simulate_batch_array(self, arr, n_packets, seed=0, streams=None, first_packet_ids=None, **kw):
+        s, q, n = super().simulate_batch_array(arr, n_packets, seed, streams, first_packet_ids, **kw)
+        if len(self.launches) == 1:
+            s[:, 1] *= 1.6
+            q[:, 1] *= 1.6
+        return s, q, n
+
+
 def test_adaptive_sweep_plan_and_statistics():
     """The pilot estimates R = R_d + R_s and its per-packet variance; each
     point gets N = clamp(ceil(Var / (R rse)^2), n_min, n_max) packets, the
@@ -117,3 +130,27 @@ def test_adaptive_sweep_plan_and_statistics():
             assert o["rse_R"] <= rse * 1.000001 or need[k] == n_max
         else:
             assert o["rse_R"] == float("inf") and need[k] == n_max
+
+
+def test_adaptive_sweep_tops_up_points_short_of_the_target():
+    """After the main launch a point whose measured RSE of R is above the
+    target (and below the cap) gets its shortfall, ceil(margin Var /
+    (R rse)^2) - N, in a further launch on the packet ids after its last
+    ones; every uncapped point then meets the target."""
+    from ismc_amd.workloads import lookup_grid
+    _, _, cfgs = lookup_grid(2, 3)
+    p_R = np.array([0.3, 0.05, 0.5, 0.9, 0.6, 0.01])
+    eng = BiasedPilotEngine(p_R)
+    rse, n_max = 0.01, 200_000
+    out, info = lookup.adaptive_sweep(cfgs, rse=rse, pilot=1000, n_min=1000, n_max=n_max, seed=3, engine=eng)
+    assert info["topup_launches"] >= 1 and len(eng.launches) == 2 + info["topup_launches"]
+    main = eng.launches[1]
+    top = eng.launches[2]
+    done = 1000 + main["n"].astype(np.int64)
+    assert np.all(top["first"] == done)
+    assert info["topup_packets"] == sum(int(l["n"].sum()) for l in eng.launches[2:])
+    for k, o in enumerate(out):
+        assert o["n_packets"] == 1000 + sum(int(l["n"][k]) for l in eng.launches[1:])
+        assert o["rse_R"] <= rse * (1 + 1e-9) or o["n_packets"] == n_max
+    assert info["max_rse_R_uncapped"] <= rse * (1 + 1e-9)
+    assert info["total_packets"] == sum(o["n_packets"] for o in out)
