@@ -389,6 +389,103 @@ def solve_plain(problem_, guess=GENERIC_GUESS, packets=1_000_000, step=1.0, tol=
     return sol[0], int(nev[0]), time.perf_counter() - t0, st
 
 
+# minimize.o's accept window for the synthetic measurements' box (truth mu_a
+# 0.01..10 /cm, mu_s' 0.2..200 /cm), a factor of two wider on every side
+PLAIN_WINDOW = ((0.005, 20.0), (0.1, 400.0))
+
+
+def _plain_stream(stream_base, p, count, r, runs, attempt):
+    # run r's attempt of problem p; run 0's first attempt is stream_base + p,
+    # the stream the single-run plain solver has always used
+    return stream_base + p + count * (r + runs * attempt)
+
+
+def plain_batch(problems, runs=1, packets=1_000_000, tol=3e-3, engine: Optional[Engine] = None,
+                stream_base=120_000_000, window=PLAIN_WINDOW, max_retries=5, guess=GENERIC_GUESS,
+                log=None):
+    """The plain solver on every problem at once, in lock step: `runs`
+    PRAXIS minimizations per problem from the generic guess, each on its
+    own Philox stream, a result outside `window` retried on a new stream
+    (at most `max_retries` times, then kept), and the mean of the accepted
+    runs as the solution (minimize.o's loop, CUDA/ISMLMC-minimization.cu:
+    467-512; with runs=1 and no retry, solve_plain).  Each PRAXIS sees the
+    same cost values as when run alone (common random numbers per stream,
+    tallies independent of the launch), so every solution equals the
+    sequential solver's bit for bit; only the time differs.  Returns
+    (solutions [P, 2], evaluations per problem, retries per problem)."""
+    eng = engine or default_engine()
+    P = len(problems)
+    accepted = [[None] * runs for _ in range(P)]
+    nev = np.zeros(P, dtype=np.int64)
+    retries = np.zeros((P, runs), dtype=np.int64)   # per run, as minimize.o retries each run
+    todo = [(p, r) for p in range(P) for r in range(runs)]
+    attempt = 0
+    (a0, a1), (s0, s1) = window
+    while todo:
+        # one lock-step batch per attempt; streams per (problem, run, attempt)
+        cost_probs = [problems[p] for p, _ in todo]
+        streams = np.array([_plain_stream(stream_base, p, P, r, runs, attempt) for p, r in todo])
+        x0 = [np.log(np.asarray(guess, dtype=np.float64))] * len(todo)
+        cost = LogCost(eng, cost_probs, packets, seed=3, streams=streams)
+        res = _praxis.minimize_batch(x0, cost, tol, 1.0, max_evals=400)
+        nxt = []
+        for k, ((p, r), (x, fx, nf)) in enumerate(zip(todo, res)):
+            xs = np.exp(x)
+            nev[p] += nf
+            if (a0 <= xs[0] <= a1 and s0 <= xs[1] <= s1) or retries[p, r] >= max_retries:
+                accepted[p][r] = xs
+            else:
+                retries[p, r] += 1
+                nxt.append((p, r))
+        if log:
+            log("    plain (runs %d), attempt %d: %d solves, %d to retry" % (runs, attempt, len(todo), len(nxt)))
+        todo = nxt
+        attempt += 1
+    sol = np.array([np.mean(a, axis=0) for a in accepted])
+    return sol, nev, retries.sum(axis=1)
+
+
+def plain_sequential(problem_, p, count, runs=1, packets=1_000_000, tol=3e-3, engine=None,
+                     stream_base=120_000_000, window=PLAIN_WINDOW, max_retries=5):
+    """One problem (index p of `count`) through plain_batch's loop one
+    PRAXIS and one launch per evaluation at a time, as minimize.o runs it.
+    Returns (solution, evaluations, seconds)."""
+    t0 = time.perf_counter()
+    (a0, a1), (s0, s1) = window
+    acc, nev = [], 0
+    for r in range(runs):
+        attempt = 0
+        while True:
+            x, nf, _, _ = solve_plain(problem_, packets=packets, tol=tol, engine=engine,
+                                      stream=_plain_stream(stream_base, p, count, r, runs, attempt))
+            nev += nf
+            if (a0 <= x[0] <= a1 and s0 <= x[1] <= s1) or attempt >= max_retries:
+                break
+            attempt += 1
+        acc.append(x)
+    return np.mean(acc, axis=0), nev, time.perf_counter() - t0
+
+
+def wilson(k, n, z=1.96):
+    """Wilson score interval of a binomial proportion k / n."""
+    if n <= 0:
+        return [0.0, 1.0]
+    p = k / n
+    d = 1 + z * z / n
+    c = (p + z * z / (2 * n)) / d
+    h = z * math.sqrt(p * (1 - p) / n + z * z / (4 * n * n)) / d
+    return [max(0.0, c - h), min(1.0, c + h)]
+
+
+def ratio_ci(k1, n1, k2, n2, z=1.96):
+    """95 % interval of (k1/n1) / (k2/n2) for independent binomial counts
+    (Katz's log method; 0.5 added to a zero count)."""
+    a, b = (k1 or 0.5), (k2 or 0.5)
+    r = (a / n1) / (b / n2)
+    se = math.sqrt(max(0.0, 1 / a - 1 / n1 + 1 / b - 1 / n2))
+    return [r * math.exp(-z * se), r * math.exp(z * se)]
+
+
 def errors(sol, truth_mua, truth_musr):
     ea = np.abs(sol[:, 0] - truth_mua) / truth_mua
     es = np.abs(sol[:, 1] - truth_musr) / truth_musr
@@ -459,11 +556,17 @@ def fit_cost(problems, sol, packets, eng, seed=13):
 def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_packets=1_000_000,
         plain_count=8, plain_packets=1_000_000, seed=0, engine: Optional[Engine] = None,
         methods=("newton", "praxis"), with_ablation=True, log=print, noise=0.01, plain_tol=3e-3,
-        fallback="newton"):
+        fallback="newton", plain=None):
     """Surrogate-seeded batched solves vs plain MC minimization on the same
     GPU, over `count` synthetic measurements (`noise` relative Gaussian
-    noise on R_d and T_d)."""
+    noise on R_d and T_d).  `plain` lists (runs, accuracy sample, timed
+    sample) triples of the plain solver (default: one run, `plain_count`
+    measurements for both); entry "plain" is the one-run solver, the
+    others "plain_runsN".  Every within-1 % rate carries a Wilson 95 %
+    interval."""
     eng = engine or default_engine()
+    if plain is None:
+        plain = ((1, plain_count, plain_count),)
     meas = make_measurements(count, meas_packets, noise=noise, seed=seed, engine=eng)
     probs = [problem(r, t) for r, t in zip(meas["R_d"], meas["T_d"])]
     out = {"count": count, "noise": noise, "meas_packets": meas_packets, "refine_packets": refine_packets,
@@ -484,7 +587,10 @@ def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_p
         info["seconds"] = total
         info["solves_per_s"] = count / total
         info["seed_errors"] = summary(*errors(seeds, meas["mu_a"], meas["mus_reduced"]))
-        info["errors"] = summary(*errors(sol, meas["mu_a"], meas["mus_reduced"]))
+        ea, es = errors(sol, meas["mu_a"], meas["mus_reduced"])
+        info["errors"] = summary(ea, es)
+        k_ok = int(np.sum((ea <= 0.01) & (es <= 0.01)))
+        info["within_1pct"] = {"count": k_ok, "of": count, "frac": k_ok / count, "ci95": wilson(k_ok, count)}
         c = fit_cost(probs, sol, check_packets, eng)
         info["fits_as_well_as_truth"] = float(np.mean(c <= cost_truth + FIT_TOL))
         out["seeded_" + method] = info
@@ -502,21 +608,41 @@ def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_p
                                    "fits_as_well_as_truth": float(np.mean(c2 <= cost_truth + FIT_TOL)),
                                    "errors": summary(*errors(sol2, meas["mu_a"], meas["mus_reduced"]))}
         log("  unseeded: %.1f solves/s, %.1f evals/solve" % (count / t2, nev2.mean()))
-    pe, ptimes, pevals = [], [], []
-    for k in range(plain_count):
-        x, nf, secs, _ = solve_plain(probs[k], packets=plain_packets, tol=plain_tol, engine=eng,
-                                     stream=120_000_000 + k)
-        pe.append(x)
-        ptimes.append(secs)
-        pevals.append(nf)
-        log("  plain solve %d: %.2f s, %d evaluations" % (k, secs, nf))
-    pe = np.array(pe)
-    c3 = fit_cost(probs[:plain_count], pe, check_packets, eng)
-    out["plain"] = {"solves_per_s": plain_count / sum(ptimes), "seconds": sum(ptimes),
-                    "fits_as_well_as_truth": float(np.mean(c3 <= cost_truth[:plain_count] + FIT_TOL)),
-                    "count": plain_count, "mean_evals": float(np.mean(pevals)),
-                    "packets_per_eval": plain_packets,
-                    "errors": summary(*errors(pe, meas["mu_a"][:plain_count], meas["mus_reduced"][:plain_count]))}
+    for runs, count_p, timed in plain:
+        key = "plain" if runs == 1 else "plain_runs%d" % runs
+        out[key] = _plain_stats(probs, meas, cost_truth, runs, min(count_p, count), min(timed, count_p),
+                                plain_packets, plain_tol, check_packets, eng, log)
     for method in methods:
         out["speedup_%s_vs_plain" % method] = out["seeded_" + method]["solves_per_s"] / out["plain"]["solves_per_s"]
     return out
+
+
+def _plain_stats(probs, meas, cost_truth, runs, count, timed, packets, tol, check_packets, eng, log):
+    """Accuracy of the plain solver over the first `count` measurements (run
+    in lock step, plain_batch) and its solves/s from the first `timed` of
+    them solved one evaluation at a time (plain_sequential), whose
+    solutions must equal the lock-step ones."""
+    t0 = time.perf_counter()
+    sol, nev, retries = plain_batch(probs[:count], runs=runs, packets=packets, tol=tol, engine=eng, log=log)
+    t_batch = time.perf_counter() - t0
+    seq, secs, sevals = [], [], []
+    for k in range(timed):
+        x, nf, dt = plain_sequential(probs[k], k, count, runs=runs, packets=packets, tol=tol, engine=eng)
+        seq.append(x)
+        secs.append(dt)
+        sevals.append(nf)
+        log("  plain (runs %d) solve %d: %.2f s, %d evaluations" % (runs, k, dt, nf))
+    same = bool(np.array_equal(np.array(seq), sol[:timed])) if timed else None
+    c = fit_cost(probs[:count], sol, check_packets, eng)
+    ea, es = errors(sol, meas["mu_a"][:count], meas["mus_reduced"][:count])
+    k_ok = int(np.sum((ea <= 0.01) & (es <= 0.01)))
+    return {"runs": runs, "count": count, "timed": timed,
+            "timed_as": "sequential: one PRAXIS per run, one %d-packet launch per cost evaluation, from "
+                        "the generic guess%s" % (packets, "; mean of %d accepted runs" % runs if runs > 1 else ""),
+            "solves_per_s": timed / sum(secs) if timed else None, "seconds_timed": sum(secs),
+            "mean_evals_timed": float(np.mean(sevals)) if timed else None,
+            "mean_evals": float(np.mean(nev)), "retries": int(retries.sum()),
+            "sequential_equals_lockstep": same, "lockstep_s": t_batch, "packets_per_eval": packets,
+            "fits_as_well_as_truth": float(np.mean(c <= cost_truth[:count] + FIT_TOL)),
+            "within_1pct": {"count": k_ok, "of": count, "frac": k_ok / count, "ci95": wilson(k_ok, count)},
+            "errors": summary(ea, es)}

@@ -64,14 +64,18 @@ def sample_config(mu_a, mu_s, g, n, t):
 
 
 def generate_dataset(n_samples, packets, seed=0, engine: Optional[Engine] = None,
-                     points_per_launch=16384, progress=None, autotune=True):
+                     points_per_launch=16384, progress=None, autotune=True, return_se=False):
     """MC labels for n_samples inputs: one batched launch per
     points_per_launch samples (one work-queue chunk per sample unless
     `autotune` picks the launch shape; labels are identical either way).
-    Returns (inputs dict, tallies [n, 4] in OUTPUTS order, seconds, packets)."""
+    Returns (inputs dict, tallies [n, 4] in OUTPUTS order, seconds, packets),
+    with return_se the labels' MC standard errors [n, 4] before the seconds
+    (R_tot's from E[R_d^2] + E[R_s^2]: a packet leaves its weight in one
+    exit tally)."""
     eng = engine or default_engine()
     inp = sample_inputs(n_samples, seed)
     tal = np.zeros((n_samples, 4))
+    se = np.zeros((n_samples, 4))
     t0 = time.perf_counter()
     for s in range(0, n_samples, points_per_launch):
         e = min(n_samples, s + points_per_launch)
@@ -83,8 +87,15 @@ def generate_dataset(n_samples, packets, seed=0, engine: Optional[Engine] = None
         for k, r in enumerate(res):
             m = r.mean
             tal[s + k] = (m["R_d"], m["T_d"], m["T_u"], m["R_d"] + m["R_s"])
+            if return_se:
+                N = float(r.n_packets)
+                rt = m["R_d"] + m["R_s"]
+                v_tot = max((r.sumsq["R_d"] + r.sumsq["R_s"]) / N - rt * rt, 0.0)
+                se[s + k] = (r.se["R_d"], r.se["T_d"], r.se["T_u"], (v_tot / max(N - 1, 1)) ** 0.5)
         if progress:
             progress(e, n_samples, time.perf_counter() - t0)
+    if return_se:
+        return inp, tal, se, time.perf_counter() - t0, n_samples * int(packets)
     return inp, tal, time.perf_counter() - t0, n_samples * int(packets)
 
 
@@ -301,3 +312,48 @@ def train_surrogate(inputs, tallies, epochs=60, batch=4096, lr=2e-3, seed=0, hol
             "n_train": int(tr.size), "n_holdout": int(hold.size), "holdout_rmse": err,
             "holdout_median_rel": rel}
     return Surrogate(mlp, info), info
+
+
+HOLDOUT_SEED = 20_000_003   # input draws and Philox seed of the fresh held-out set
+
+
+def holdout_report(sur, n_samples=16384, packets=100_000, seed=HOLDOUT_SEED, engine: Optional[Engine] = None):
+    """The network against fresh MC labels: n_samples inputs drawn over the
+    training box with a seed the training never used, each labelled by
+    `packets` packets of this engine.  Per output: RMSE of the prediction
+    against the labels, the labels' median and RMS MC standard error, their
+    ratio, and the network's own RMSE with the labels' noise removed,
+    sqrt(max(RMSE^2 - mean SE^2, 0)).  Also the label generation's
+    packets/s."""
+    eng = engine or default_engine()
+    inp, tal, se, secs, npk = generate_dataset(n_samples, packets, seed=seed, engine=eng, return_se=True)
+    p = sur.predict(inp["mu_a"], inp["mu_s"], inp["g"], inp["n"], inp["t"])
+    out = {}
+    for i, k in enumerate(OUTPUTS):
+        d = p[k] - tal[:, i]
+        rmse = float(np.sqrt(np.mean(d * d)))
+        msq = float(np.mean(se[:, i] ** 2))
+        med = float(np.median(se[:, i]))
+        out[k] = {"rmse": rmse, "median_label_se": med, "rms_label_se": msq ** 0.5,
+                  "rmse_over_median_se": rmse / med if med > 0 else None,
+                  "rmse_over_rms_se": rmse / msq ** 0.5 if msq > 0 else None,
+                  "network_rmse_noise_removed": max(rmse * rmse - msq, 0.0) ** 0.5}
+    return {"samples": n_samples, "packets_per_label": int(packets), "seed": seed,
+            "label_packets": int(npk), "label_seconds": secs, "label_packets_per_s": npk / secs,
+            "outputs": out, "inputs": inp, "labels": tal, "label_se": se}
+
+
+def time_training(x, y, epochs=10, batch=4096, device=0, seed=0):
+    """Device ms per Adam step of the network's training on (x, y) (a fresh
+    network; the packaged one is not touched): `epochs` passes of
+    len(x) // batch steps in one ismc_mlp_train call."""
+    mlp = MLP(max_batch=batch, device=device)
+    try:
+        mlp.set_params(init_params(mlp.arch, seed))
+        mlp.load_dataset(x, y)
+        steps = x.shape[0] // batch
+        losses, ms = mlp.train(epochs, batch=batch, lr=1e-3, seed=seed)
+        return {"steps": epochs * steps, "batch": batch, "ms_per_step": ms / (epochs * steps),
+                "loss_first_last": [float(losses[0]), float(losses[-1])]}
+    finally:
+        mlp.close()

@@ -107,3 +107,48 @@ def test_surrogate_pipeline_small(engine):
     print("holdout", info["holdout_rmse"], "train ms", info["train_ms"])
     assert info["losses"][-1] < info["losses"][0]
     assert info["holdout_rmse"]["R_d"] < 0.05 and info["holdout_rmse"]["T_d"] < 0.05
+
+
+def test_train_rejects_indices_outside_the_dataset():
+    """ismc_mlp_train checks every batch index against the resident dataset
+    on the host (an index outside it would make the gather read beyond the
+    device buffers): negative or too-large indices give ISMC_EINVAL before
+    any launch, and the network trains normally afterwards."""
+    import ctypes
+    m = S.MLP(max_batch=128)
+    try:
+        m.set_params(S.init_params(m.arch, 3))
+        rng = np.random.default_rng(0)
+        x = rng.uniform(-1, 1, (256, 5)).astype(np.float32)
+        y = rng.normal(0, 1, (256, 4)).astype(np.float32)
+        m.load_dataset(x, y)
+        L = m._L
+        losses = np.zeros(1, dtype=np.float32)
+        ms = ctypes.c_float()
+        for bad in (-1, 256, 1 << 30):
+            perm = np.arange(256, dtype=np.int32)
+            perm[100] = bad
+            rc = L.ismc_mlp_train(m._h, perm.ctypes.data, 1, 2, 128, 1e-3, 0.9, 0.999, 1e-8,
+                                  losses.ctypes.data, ctypes.byref(ms))
+            assert rc == -1 and b"index" in L.ismc_mlp_last_error(m._h)
+        before = m.get_params()
+        l, _ = m.train(2, batch=128, lr=1e-3)
+        assert np.all(np.isfinite(l)) and not np.array_equal(before, m.get_params())
+    finally:
+        m.close()
+
+
+def test_packaged_network_against_fresh_labels(engine):
+    """The packaged network against 2^14 fresh samples of the training box,
+    labelled by 1e5 MC packets each (a seed the training never used): its
+    RMSE on the sphere readings R_d and T_d stays within 1.5 times the
+    labels' median MC standard error, and with the labels' own noise
+    removed the network's error is within 1.25 times that standard error."""
+    sur = S.Surrogate.load(S.packaged_weights())
+    rep = S.holdout_report(sur, n_samples=1 << 14, packets=100_000, engine=engine)
+    o = rep["outputs"]
+    print({k: {q: round(v, 6) for q, v in o[k].items() if v is not None} for k in o},
+          "%.3g label packets/s" % rep["label_packets_per_s"])
+    for k in ("R_d", "T_d"):
+        assert o[k]["rmse"] <= 1.5 * o[k]["median_label_se"], (k, o[k])
+        assert o[k]["network_rmse_noise_removed"] <= 1.25 * o[k]["median_label_se"], (k, o[k])

@@ -7,7 +7,8 @@ import pytest
 
 from helpers import golden_cases, load_golden, spec_of, vec, zscores
 from ismc_amd import ISMC, ISMLMC, Layer, SampleConfig, Sphere
-from ismc_amd.workloads import single_layer, slab_between_slides, three_layer_sample
+from ismc_amd.workloads import (lookup_example, single_layer, slab_between_slides, three_layer_sample,
+                                wang_three_layer_example)
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -28,6 +29,8 @@ def _cases():
     yield "iso_glass_behind", SampleConfig(
         layers=[Layer(1.33, 0.0, 0.05, 0.3, 20.0), Layer.glass(1.5, 0.2)],
         R=slab_between_slides().R, T=slab_between_slides().T)
+    # the reference's multilayer --example (CUDA/ISMLMC.cu:123-157)
+    yield "wang_example", wang_three_layer_example()
 
 
 CASES = list(_cases())
@@ -364,3 +367,47 @@ def test_error_returns_and_zero_packets(engine):
     assert z.n_packets == 0 and all(v == 0 for v in z.sum.values())
     ok = engine.simulate(cfg, 5000, seed=3)
     assert ok.n_packets == 5000 and 0.99 < sum(ok.mean.values()) < 1.01
+
+
+def test_reference_multilayer_example(engine):
+    """The reference's multilayer --example (CUDA/ISMLMC.cu:123-157): Wang
+    et al.'s three index-matched layers (n 1.37) with an isotropic middle
+    layer between the example spheres, 1e6 packets: every tally and the
+    event counts per packet agree with the oracle within 3 combined SE."""
+    cfg = wang_three_layer_example()
+    assert [L.mu_s for L in cfg.layers] == pytest.approx([100.0, 10.0, 10.0])
+    n = 1_000_000
+    g = engine.simulate(cfg, n, seed=41)
+    o = O.simulate(spec_of(cfg), n, seed=42, threads=8)
+    z = zscores(vec(g.mean), vec(g.se), vec(o["mean"]), vec(o["se"]))
+    print("wang example gpu", np.round(vec(g.mean), 5), "z", np.round(z, 2))
+    assert g.n_aborted == 0 and np.all(np.abs(z) < 3.0)
+    assert sum(g.mean.values()) == pytest.approx(1.0, abs=1e-3)
+
+
+def test_reference_lookup_example(engine, tmp_path):
+    """The reference's lookup --example (CUDA/ISMLMC-lookup.cu:576-591):
+    13 x 11 grid, 1e5 packets x 10 runs, separate R-sphere and T-sphere
+    simulations, in one launch.  Sampled points' R_d (R-sphere run) and T_d
+    (T-sphere run) agree with the oracle at 1e6 packets within 3 combined
+    SE (the sweep's SE taken from the oracle's per-packet variance, both
+    estimating the same distribution with 1e6 packets), the reported
+    standard errors are the runs' population std / sqrt(10), and the CSV
+    tables read back."""
+    from ismc_amd import lookup
+    from ismc_amd.lookup import _grid_configs
+    ex = lookup_example()
+    res = lookup.sample_space(**ex, seed=51, engine=engine)
+    assert res.R_d.shape == (13, 11) and res.total_packets == 13 * 11 * 10 * 2 * 100_000
+    assert np.allclose(res.R_d_unc, res.runs_R_d.std(axis=0) / np.sqrt(10), rtol=1e-12)
+    for i, j in ((0, 0), (6, 5), (12, 10)):
+        rc, tc = _grid_configs(ex["n"], ex["g"], ex["t"], [ex["mua"][i]], [ex["mus_reduced"][j]], ex["R"],
+                               ex["T"], False, True, 1e-4)
+        for tally, cfg, got in (("R_d", rc, res.R_d[i, j]), ("T_d", tc, res.T_d[i, j])):
+            o = O.simulate(spec_of(cfg), 1_000_000, seed=52 + i, threads=8)
+            z = (got - o["mean"][tally]) / (np.sqrt(2.0) * o["se"][tally])
+            print("lookup example", i, j, tally, got, o["mean"][tally], "z %.2f" % z)
+            assert abs(z) < 3.0
+    pr, pt = lookup.write_tables(res, str(tmp_path))
+    m, s = lookup.read_csv_table(pr, 13)
+    assert np.allclose(m, res.R_d, rtol=1e-5) and np.allclose(s, res.R_d_unc, rtol=1e-5)

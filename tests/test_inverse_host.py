@@ -138,3 +138,65 @@ def test_predict_log10_equals_predict():
     assert np.max(np.abs(sur.mlp.seen[0] - sur.mlp.seen[1])) <= 2 ** -22
     for k in b:
         assert np.allclose(a[k], b[k], rtol=0, atol=1e-6)
+
+
+class NoisyStreamEngine(AnalyticEngine):
+    """The analytic model plus a smooth offset that depends on each point's
+    Philox stream (as MC noise does): runs on different streams find
+    different minima, so the plain solver's stream bookkeeping shows."""
+
+    def simulate_batch_array(self, cfg, n, seed=0, streams=None, use_graph=False, **kw):
+        s, q, npk = super().simulate_batch_array(cfg, n, seed, streams, use_graph, **kw)
+        st = np.asarray(streams, dtype=np.float64)
+        f = 1.0 + 0.02 * np.sin(0.7 * st)
+        s = s.copy()
+        s[:, 1] *= f
+        s[:, 3] *= 2.0 - f
+        return s, s.copy(), npk
+
+
+def test_plain_lockstep_equals_sequential_with_runs_and_retries():
+    """minimize.o's loop (runs averaged, a result outside the accept window
+    retried on a new stream, at most max_retries times per run) gives the
+    same solutions run in lock step as one evaluation at a time."""
+    eng = NoisyStreamEngine()
+    truths = np.array([(0.05, 2.0), (0.5, 20.0), (3.0, 80.0)])
+    probs = []
+    for a, s in truths:
+        t = AnalyticEngine.tallies(inverse.problem(0.5, 0.5).configs(a, s)[0][0])
+        probs.append(inverse.problem(t["R_d"], t["T_d"]))
+    # a window that part of the first problem's runs miss (its mu_a is poorly
+    # determined under the stream-dependent offset)
+    window = ((0.1, 20.0), (0.1, 400.0))
+    sol, nev, retries = inverse.plain_batch(probs, runs=3, packets=1000, engine=eng, window=window,
+                                            max_retries=2)
+    # the loop written out: each run retried on its next stream until inside
+    # the window or out of retries, the accepted runs averaged
+    for p in range(len(probs)):
+        acc, n_retry = [], 0
+        for r in range(3):
+            for att in range(3):
+                x, _, _, _ = inverse.solve_plain(probs[p], packets=1000, engine=eng,
+                                                 stream=inverse._plain_stream(120_000_000, p, 3, r, 3, att))
+                if (0.1 <= x[0] <= 20.0 and 0.1 <= x[1] <= 400.0) or att == 2:
+                    break
+                n_retry += 1
+            acc.append(x)
+        assert np.array_equal(np.mean(acc, axis=0), sol[p]) and retries[p] == n_retry
+        x, n, _ = inverse.plain_sequential(probs[p], p, len(probs), runs=3, packets=1000, engine=eng,
+                                           window=window, max_retries=2)
+        assert np.array_equal(x, sol[p]) and n == nev[p]
+    assert retries[0] > 0
+    # runs on different streams differ, and their mean is the solution
+    one, _, _ = inverse.plain_batch(probs, runs=1, packets=1000, engine=eng, window=window, max_retries=2)
+    assert not np.array_equal(one, sol)
+
+
+def test_wilson_and_ratio_intervals():
+    lo, hi = inverse.wilson(3, 96)
+    assert 0.0106 < lo < 0.0108 and 0.0878 < hi < 0.0880       # 3/96: [1.07 %, 8.79 %]
+    assert inverse.wilson(0, 50)[0] == 0.0 and inverse.wilson(50, 50)[1] == 1.0
+    lo, hi = inverse.ratio_ci(339, 4096, 35, 512)
+    r = (339 / 4096) / (35 / 512)
+    # Katz: exp(+-1.96 sqrt(1/339 - 1/4096 + 1/35 - 1/512)) = 0.7149, 1.3988
+    assert abs(lo / r - 0.7149) < 1e-3 and abs(hi / r - 1.3988) < 1e-3
