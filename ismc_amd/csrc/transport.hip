@@ -30,7 +30,11 @@
 //    theta, cos phi).  x, y, mu_x, mu_y (and sin phi) are dead state in the
 //    reference and are not computed.
 //  * Random numbers: Philox4x32-10 (Salmon et al. SC'11), key = seed,
-//    counter = (packet id lo, packet id hi, event index, stream).  Each
+//    counter = (packet id lo, event index, packet id hi, stream).  The event
+//    index sits in the counter word that the first round does not
+//    multiply, so four of a block's twenty 32 x 32 -> 64 products depend on
+//    the packet alone and are taken once, when a lane takes the packet
+//    (philox_fixed); each event's block then costs sixteen.  Each
 //    event draws one 4-word block.  Words are used as
 //      entrance:     w0 reflect test, w1 first step (or return draw), w2 new |mu_z|
 //      interaction:  w0 next step, w1 HG, w2 azimuth, w3 roulette
@@ -205,7 +209,9 @@ template <> struct DM<double> {
   }
 };
 
-using ismc_rng::philox4x32_10;
+using ismc_rng::PhiloxFixed;
+using ismc_rng::philox_block;
+using ismc_rng::philox_fixed;
 
 // 23-bit uniform 1 - (x >> 9) 2^-23 in (0, 1], exact in float and double.
 // One v_alignbit_b32 shifts the word's top 23 bits under the exponent of
@@ -346,6 +352,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
   R bu0 = 0;           // Fresnel draw of a pending interface hit
   uint64_t pid = 0, outidx = 0;
   uint32_t ev = 0, stream = 0;
+  PhiloxFixed pf{0u, 0u, 0u, 0u, 0u};  // the packet's event-independent Philox products
   uint32_t e_int = 0, e_bnd = 0, e_inc = 0, e_sph = 0, e_rei = 0;
   // current layer cache
   R L_zlo = 0, L_zhi = 0, L_step = 0, L_abs = 0, L_ha = 0, L_hb = 0, L_hc = 0, L_hd = 0,
@@ -497,6 +504,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
           pid = win_next + rank;
           set = cur_set;
           stream = win_stream;
+          pf = philox_fixed((uint32_t)pid, (uint32_t)(pid >> 32), stream, k0, k1);
           outidx = win_outbase + (pid - win_first);
           w = (R)1;
           muz = pts[set].mu0_z;
@@ -572,7 +580,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     const bool run_enter = step == 0 && rare && (phase == PH_ENTER);
     if (__ballot(run_flight || run_enter) == 0) break;
     uint32_t rw[4];
-    philox4x32_10((uint32_t)pid, (uint32_t)(pid >> 32), ev, stream, k0, k1, rw);
+    philox_block(pf, ev, k0, k1, rw);
     // The event's first three uniforms go straight into the registers an
     // interface hit or a reflected entrance keeps them in (bu0: Fresnel
     // draw; su1, su2: return draw and new |mu_z|).  Every lane that runs

@@ -18,8 +18,8 @@
  *
  * Random numbers: the reference draws from cuRAND XORWOW / Python MT19937.
  * The oracle (like the GPU path) draws from Philox4x32-10 (Salmon et al.,
- * SC'11) with key = seed and counter = (packet id lo, packet id hi, event
- * index, stream id); every event of a packet gets one 4-word block and the
+ * SC'11) with key = seed and counter = (packet id lo, event index, packet
+ * id hi, stream id); every event of a packet gets one 4-word block and the
  * reference's draw sites map onto fixed words of that block (see
  * ismc_oracle.c).  Parity with the reference is therefore statistical, and
  * parity between this oracle and the GPU kernel is per packet.

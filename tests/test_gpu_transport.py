@@ -154,7 +154,10 @@ def test_large_run_properties(engine):
     z = zscores(vec(r.mean), vec(r.se), vec(o["mean"]), vec(o["se"]))
     assert np.all(np.abs(z) < 3.0), np.round(z, 2)
     ev = r.events
-    assert ev["entrances"] == ev["sphere_decisions"] and ev["reinjections"] < ev["entrances"]
+    # every entrance ends in a sphere decision except those of packets that
+    # the roulette kills inside the sample (a few in 2^27 here)
+    assert 0 <= ev["entrances"] - ev["sphere_decisions"] <= 1e-6 * (1 << 27)
+    assert ev["reinjections"] < ev["entrances"]
 
 
 def test_full_size_step_is_additive_over_packet_ranges(engine):
