@@ -1,16 +1,36 @@
-// Surrogate MLP for the inverse solver: x (5 features) -> [256 SiLU] x 4 -> 4
-// logit-transformed tallies, trained on this Monte Carlo's own output.
+// Surrogate MLP for the inverse solver: x (5 features) -> [hidden SiLU] x depth
+// -> 4 logit-transformed tallies, trained on this Monte Carlo's own output.
+// There is no counterpart in the reference: it seeds the minimization solver.
 //
-// All contractions run on the gfx950 matrix cores: BF16 operands, FP32
-// accumulation (v_mfma_f32_16x16x32_bf16).  Master weights and Adam state
-// are FP32; a BF16 copy of the weights feeds the MFMA.  There is no
-// counterpart in the reference: it seeds the minimization solver.
+// Every contraction runs on the gfx950 matrix cores: v_mfma_f32_16x16x32_bf16,
+// BF16 operands, FP32 accumulation.  Master weights and Adam state are FP32;
+// two BF16 copies of every weight matrix feed the MFMA, W^T [out][in] for
+// the forward and W [in][out] for the backward, so that each lane's B
+// fragment (8 consecutive k of one column) is one 16-byte load in both.
 //
-// GEMM: C[M,N] = op(A)[M,K] * op(B)[K,N], 64x64 block tile, K step 32,
-// four waves per block each owning a 32x32 quarter as 2x2 16x16 MFMA tiles.
-// Tiles go through LDS stored K-contiguous (As[m][k], Bs[n][k]), so each
-// lane's MFMA fragment (8 consecutive k of one row / column) is one 16-byte
-// LDS read.  Transposed operands are handled in the global->LDS load.
+// A training step is four launches:
+//   1. mlp_forward   one block per 16 batch rows runs all layers: the rows'
+//                    activations stay in LDS (A fragments), the weights come
+//                    from L2 as B fragments; each hidden layer's epilogue
+//                    adds the bias and applies SiLU, keeps the pre-activation
+//                    (fragment order) and writes the activation transposed
+//                    ([unit][row], the weight-gradient kernel's A operand);
+//                    the output layer's epilogue forms the MSE gradient and
+//                    the block's squared-error sum.
+//   2. mlp_backward  the same row blocks run the gradient back through the
+//                    layers (dZ W^T, times SiLU'), writing every layer's dZ
+//                    transposed ([unit][row], the weight-gradient kernel's B
+//                    operand).
+//   3. mlp_dweight   every layer's weight gradient A^T dZ at once: one wave
+//                    per 32 x 32 tile of one layer and one slice of the
+//                    batch rows; the waves of each layer's first tile row
+//                    also sum their dZ fragments (the bias gradient); one
+//                    slab of partial sums per row slice.
+//   4. mlp_adam      each parameter sums its slabs in a fixed order and takes
+//                    one Adam step, refreshing both BF16 copies.
+// Every sum has a fixed order, so training is deterministic.  Inputs and
+// outputs are padded to 32 (one MFMA K step each), with the padding masked
+// out of Adam.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,8 +49,11 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16_t;
 
-constexpr int TM = 64, TN = 64, TK = 32;
-constexpr int LDK = TK + 8;  // padded LDS row (bf16 elements)
+constexpr int kRows = 16;       // batch rows per block of the fused kernels
+constexpr int kInPad = 32;      // input features padded to one MFMA K step
+constexpr int kOutPad = 32;     // outputs padded to one MFMA K step (the backward's first)
+constexpr int kMaxLayers = 17;  // at most 16 hidden layers and the output layer
+constexpr int kMaxSlices = 8;   // batch-row slices of the weight-gradient kernel
 
 __device__ __forceinline__ bf16_t f2bf(float f) {
   // round to nearest even
@@ -38,198 +61,386 @@ __device__ __forceinline__ bf16_t f2bf(float f) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return (bf16_t)(u >> 16);
 }
-__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
-enum Epi : int {
-  EPI_STORE_F32 = 0,     // C (fp32)
-  EPI_BIAS_SILU = 1,     // Z = C + b (fp32), A = silu(Z) (bf16)
-  EPI_BIAS_F32 = 2,      // Y = C + b (fp32)
-  EPI_SILU_GRAD = 3,     // dZ = C * silu'(Zprev) (bf16)
+__device__ __forceinline__ bf16x8 load8(const bf16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+
+__device__ __forceinline__ uint2 pack4(const uint16_t h[4]) {
+  uint2 pk;
+  pk.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+  pk.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+  return pk;
+}
+
+// Device view of one network's parameters.  Layer l maps K[l] inputs to
+// N[l] outputs; in the FP32 flat buffer its W [K][N] (row-major) starts at
+// w_off[l] and its bias at b_off[l]; both BF16 copies start at wb_off[l].
+struct Net {
+  int n_in, n_out, H, depth;
+  const float* w;
+  const bf16_t* wt;  // W^T [N][K] per layer
+  const bf16_t* wb;  // W   [K][N] per layer
+  int K[kMaxLayers], N[kMaxLayers];
+  int w_off[kMaxLayers], b_off[kMaxLayers], wb_off[kMaxLayers];
 };
 
-struct GemmArgs {
-  int M, N, K;
-  const bf16_t* A;
-  int lda;
-  const bf16_t* B;
-  int ldb;
-  float* C;        // fp32 out (EPI_STORE_F32 / Z / Y), row-major ldc
-  int ldc;
-  bf16_t* Cb;      // bf16 out (A or dZ), row-major ldc
-  const float* bias;
-  const float* Zp; // pre-activation for EPI_SILU_GRAD, row-major ldc
-  int k_split;     // number of K slices (gridDim.z); C has k_split slabs of M x ldc
-};
-
-// op(A)[m][k] = A[m*lda + k] (TA = false) or A[k*lda + m] (TA = true)
-// op(B)[k][n] = B[k*ldb + n] (TB = false) or B[n*ldb + k] (TB = true)
-template <bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) bf16_t As[TM][LDK];
-  __shared__ __attribute__((aligned(16))) bf16_t Bs[TN][LDK];
-  const int t = threadIdx.x;
-  const int wave = t >> 6, lane = t & 63;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
-  const int kslice = g.K / g.k_split;
-  const int kbeg = blockIdx.z * kslice, kend = kbeg + kslice;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-
-  f32x4 acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-  for (int k0 = kbeg; k0 < kend; k0 += TK) {
-    // ---- A tile: 64 x 32 ----
-    if (!TA) {
-      const int r = t >> 2, c = (t & 3) * 8;
-      const uint4 v = *reinterpret_cast<const uint4*>(g.A + (size_t)(m0 + r) * g.lda + k0 + c);
-      *reinterpret_cast<uint4*>(&As[r][c]) = v;
-    } else {
-      const int kk = t >> 3, c = (t & 7) * 8;
-      const uint4 v = *reinterpret_cast<const uint4*>(g.A + (size_t)(k0 + kk) * g.lda + m0 + c);
-      const bf16_t* e = reinterpret_cast<const bf16_t*>(&v);
-      for (int i = 0; i < 8; ++i) As[c + i][kk] = e[i];
-    }
-    // ---- B tile: 32 x 64, stored Bs[n][k] ----
-    if (!TB) {
-      const int kk = t >> 3, c = (t & 7) * 8;
-      const uint4 v = *reinterpret_cast<const uint4*>(g.B + (size_t)(k0 + kk) * g.ldb + n0 + c);
-      const bf16_t* e = reinterpret_cast<const bf16_t*>(&v);
-      for (int i = 0; i < 8; ++i) Bs[c + i][kk] = e[i];
-    } else {
-      const int r = t >> 2, c = (t & 3) * 8;
-      const uint4 v = *reinterpret_cast<const uint4*>(g.B + (size_t)(n0 + r) * g.ldb + k0 + c);
-      *reinterpret_cast<uint4*>(&Bs[r][c]) = v;
-    }
-    __syncthreads();
-    // ---- 2x2 MFMA 16x16x32 per wave ----
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
-    bf16x8 a[2], b[2];
-    for (int i = 0; i < 2; ++i) {
-      const uint4 va = *reinterpret_cast<const uint4*>(&As[wm + i * 16 + fr][fk]);
-      a[i] = __builtin_bit_cast(bf16x8, va);
-      const uint4 vb = *reinterpret_cast<const uint4*>(&Bs[wn + i * 16 + fr][fk]);
-      b[i] = __builtin_bit_cast(bf16x8, vb);
-    }
-    for (int i = 0; i < 2; ++i)
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    __syncthreads();
+// One wave's strip of TILES 16 x 16 output tiles (columns n0 + 16 t) over
+// KSTEPS steps of 32: A fragments from the block's rows in LDS (row-major,
+// k contiguous), B fragments from global memory, Bt[n][k] (k contiguous,
+// leading dimension ldb).  Every B fragment of the strip is requested
+// before the first MFMA, so that the strip waits for one L2 latency rather
+// than one per K step (a block has one wave per SIMD or two: no other
+// wave hides the latency).  MFMA operand layout (16x16x32): lane l holds
+// row / column l % 16, k = 8 (l / 16) .. + 7; the result holds column
+// l % 16, rows 4 (l / 16) .. + 3.
+template <int TILES, int KSTEPS, int LDA>
+__device__ __forceinline__ void strip(const bf16_t (*A)[LDA], const bf16_t* __restrict__ Bt, int ldb,
+                                      int n0, int lane, f32x4 acc[TILES]) {
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  bf16x8 bv[KSTEPS][TILES];
+#pragma unroll
+  for (int s = 0; s < KSTEPS; ++s)
+#pragma unroll
+    for (int t = 0; t < TILES; ++t) bv[s][t] = load8(Bt + (size_t)(n0 + t * 16 + fr) * ldb + s * 32 + fk);
+#pragma unroll
+  for (int t = 0; t < TILES; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < KSTEPS; ++s) {
+    const bf16x8 av = load8(&A[fr][s * 32 + fk]);
+#pragma unroll
+    for (int t = 0; t < TILES; ++t)
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[s][t], acc[t], 0, 0, 0);
   }
+}
 
-  // ---- epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile ----
-  float* C = g.C ? g.C + (size_t)blockIdx.z * g.M * g.ldc : nullptr;
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j)
+// waves per block of the fused kernels: two 16-column tiles per wave of a
+// hidden layer, at most 8 waves (two per SIMD)
+template <int H> struct Waves { static constexpr int value = H / 32 < 8 ? H / 32 : 8; };
+
+struct FwdArgs {
+  Net net;
+  int B;               // rows of the batch (a multiple of kRows)
+  int n_valid;         // rows that hold data (the rest get zero inputs)
+  const float* X;      // source rows [*][n_in]
+  const float* T;      // targets [*][n_out] (training) or null
+  const int* idx;      // row -> source row (null: the row itself)
+  float* Y;            // inference: outputs [n_valid][n_out], or null
+  // training outputs, null for inference
+  float* Zf;           // pre-activations of the hidden layers, fragment order
+  bf16_t* XT;          // inputs transposed [kInPad][B]
+  bf16_t* AT;          // hidden activations transposed [depth][H][B]
+  bf16_t* dYT;         // output gradient transposed [kOutPad][B]
+  float* loss_part;    // [B / kRows] squared-error sums
+  float grad_scale;    // 2 / (B n_out)
+};
+
+// pre-activation fragment of (layer l, row block rb, column tile c): 64 lanes x 4
+__device__ __forceinline__ size_t zf_index(int l, int rb, int c, int nrb, int H, int lane) {
+  return ((((size_t)l * nrb + rb) * (H / 16) + c) * 64 + lane) * 4;
+}
+
+template <int H>
+__global__ __launch_bounds__(64 * Waves<H>::value) void mlp_forward(FwdArgs a) {
+  constexpr int LDA = H + 8;
+  constexpr int NW = Waves<H>::value;
+  constexpr int TW = H / (16 * NW);  // 16-column tiles per wave of a hidden layer
+  __shared__ __attribute__((aligned(16))) bf16_t act[2][kRows][LDA];
+  __shared__ float red[2];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int rb = blockIdx.x, row0 = rb * kRows, nrb = gridDim.x;
+  const int fr = lane & 15, cr = (lane >> 4) * 4;
+  const Net& net = a.net;
+  for (int e = tid; e < kRows * kInPad; e += 64 * NW) {
+    const int r = e / kInPad, k = e % kInPad;
+    const int row = row0 + r;
+    float v = 0.f;
+    if (k < net.n_in && row < a.n_valid) v = a.X[(size_t)(a.idx ? a.idx[row] : row) * net.n_in + k];
+    const bf16_t h = f2bf(v);
+    act[0][r][k] = h;
+    if (a.XT) a.XT[(size_t)k * a.B + row] = h;
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int l = 0; l < net.depth; ++l) {
+    const int n0 = wave * (H / NW);
+    const bf16_t* Wt = net.wt + net.wb_off[l];
+    const float* bias = net.w + net.b_off[l];
+    f32x4 acc[TW];
+    if (l == 0)
+      strip<TW, kInPad / 32, LDA>(act[cur], Wt, kInPad, n0, lane, acc);
+    else
+      strip<TW, H / 32, LDA>(act[cur], Wt, H, n0, lane, acc);
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+      const int n = n0 + t * 16 + fr;
+      const float b = bias[n];
+      f32x4 z;
+      uint16_t h[4];
+#pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
-        const int n = n0 + wn + j * 16 + (lane & 15);
-        const float v = acc[i][j][r];
-        const size_t o = (size_t)m * g.ldc + n;
-        if (EPI == EPI_STORE_F32) {
-          C[o] = v;
-        } else if (EPI == EPI_BIAS_SILU) {
-          const float z = v + g.bias[n];
-          C[o] = z;
-          g.Cb[o] = f2bf(z * sigmoid(z));
-        } else if (EPI == EPI_BIAS_F32) {
-          C[o] = v + g.bias[n];
-        } else {
-          const float z = g.Zp[o];
-          const float s = sigmoid(z);
-          g.Cb[o] = f2bf(v * s * (1.0f + z * (1.0f - s)));
-        }
+        z[r] = acc[t][r] + b;
+        h[r] = f2bf(z[r] * sigmoid(z[r]));
+        act[cur ^ 1][cr + r][n] = h[r];
       }
-}
-
-// sum of k_split slabs (fixed order) -> out; also bias gradient rows
-__global__ void reduce_slabs(const float* slabs, int k_split, size_t n, float* out) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float s = 0.f;
-  for (int k = 0; k < k_split; ++k) s += slabs[(size_t)k * n + i];
-  out[i] = s;
-}
-
-// Partial bias gradients: partial[slab][n] = sum of dZ[b][n] (bf16) over the
-// rows of one slab.  A block covers 64 columns (one row of 64 bf16 is a
-// 128-byte coalesced load) with 4 row phases; the phases are combined in a
-// fixed order, and reduce_slabs sums the slabs in a fixed order after it,
-// so the result is deterministic.
-constexpr int kBiasSlabs = 16;
-__global__ void bias_partial(const bf16_t* dZ, int B, int N, int rows_per_slab, float* partial) {
-  __shared__ float red[4][64];
-  const int c = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int n = blockIdx.x * 64 + c;
-  const int r0 = blockIdx.y * rows_per_slab;
-  const int r1 = min(B, r0 + rows_per_slab);
-  float s = 0.f;
-  if (n < N)
-    for (int b = r0 + q; b < r1; b += 4) s += bf2f(dZ[(size_t)b * N + n]);
-  red[q][c] = s;
-  __syncthreads();
-  if (q == 0 && n < N) partial[(size_t)blockIdx.y * N + n] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
-}
-
-// MSE on the first n_out columns: dY = 2 (Y - T) / (B n_out), loss partials per block
-__global__ void mse_grad(const float* Y, int ldy, const float* T, int n_out, int B, bf16_t* dY,
-                         float* loss_partial) {
-  __shared__ float red[256];
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  float l = 0.f;
-  if (b < B) {
-    for (int j = 0; j < ldy; ++j) {
+      if (a.Zf) {
+        *reinterpret_cast<f32x4*>(a.Zf + zf_index(l, rb, n0 / 16 + t, nrb, H, lane)) = z;
+        *reinterpret_cast<uint2*>(a.AT + ((size_t)l * H + n) * a.B + row0 + cr) = pack4(h);
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  // output layer: 32 padded columns, one tile each for waves 0 and 1
+  if (wave < kOutPad / 16) {
+    const int L = net.depth;
+    const int n = wave * 16 + fr;
+    f32x4 acc[1];
+    strip<1, H / 32, LDA>(act[cur], net.wt + net.wb_off[L], H, wave * 16, lane, acc);
+    const float b = net.w[net.b_off[L] + n];
+    uint16_t h[4];
+    float sq = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = row0 + cr + r;
+      const float y = acc[0][r] + b;
+      const bool live = n < net.n_out && row < a.n_valid;
+      if (a.Y && live) a.Y[(size_t)row * net.n_out + n] = y;
       float d = 0.f;
-      if (j < n_out) {
-        d = Y[(size_t)b * ldy + j] - T[(size_t)b * n_out + j];
-        l += d * d;
+      if (a.T && live) d = y - a.T[(size_t)(a.idx ? a.idx[row] : row) * net.n_out + n];
+      sq += d * d;
+      h[r] = f2bf(d * a.grad_scale);
+    }
+    if (a.dYT) *reinterpret_cast<uint2*>(a.dYT + (size_t)n * a.B + row0 + cr) = pack4(h);
+    // the wave's sum in a fixed order (butterfly), then wave 0 + wave 1
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if (lane == 0) red[wave] = sq;
+  }
+  if (a.loss_part) {
+    __syncthreads();
+    if (tid == 0) a.loss_part[rb] = red[0] + red[1];
+  }
+}
+
+struct BwdArgs {
+  Net net;
+  int B;
+  const float* Zf;
+  const bf16_t* dYT;
+  bf16_t* dZT;  // [depth][H][B]
+};
+
+template <int H>
+__global__ __launch_bounds__(64 * Waves<H>::value) void mlp_backward(BwdArgs a) {
+  constexpr int LDA = H + 8;
+  constexpr int NW = Waves<H>::value;
+  constexpr int TW = H / (16 * NW);
+  __shared__ __attribute__((aligned(16))) bf16_t g[2][kRows][LDA];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int rb = blockIdx.x, row0 = rb * kRows, nrb = gridDim.x;
+  const int fr = lane & 15, cr = (lane >> 4) * 4;
+  const Net& net = a.net;
+  for (int e = tid; e < kRows * kOutPad; e += 64 * NW) {
+    const int r = e % kRows, k = e / kRows;
+    g[0][r][k] = a.dYT[(size_t)k * a.B + row0 + r];
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int l = net.depth; l >= 1; --l) {
+    // gradient at layer l-1's output: dZ_l [rows][N_l] W_l^T, with the B
+    // fragments read from W_l [K_l][N_l] (row k_in holds one lane's N_l values)
+    const int n0 = wave * (H / NW);
+    const bf16_t* W = net.wb + net.wb_off[l];
+    f32x4 acc[TW];
+    if (l == net.depth)
+      strip<TW, kOutPad / 32, LDA>(g[cur], W, kOutPad, n0, lane, acc);
+    else
+      strip<TW, H / 32, LDA>(g[cur], W, H, n0, lane, acc);
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+      const int n = n0 + t * 16 + fr;
+      const f32x4 z = *reinterpret_cast<const f32x4*>(a.Zf + zf_index(l - 1, rb, n0 / 16 + t, nrb, H, lane));
+      uint16_t h[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float s = sigmoid(z[r]);
+        h[r] = f2bf(acc[t][r] * s * (1.0f + z[r] * (1.0f - s)));
+        g[cur ^ 1][cr + r][n] = h[r];
       }
-      dY[(size_t)b * ldy + j] = f2bf(2.0f * d / (float)(B * n_out));
+      *reinterpret_cast<uint2*>(a.dZT + ((size_t)(l - 1) * H + n) * a.B + row0 + cr) = pack4(h);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
+struct DwArgs {
+  Net net;
+  int B;
+  int slices;
+  int rows_per_slice;
+  int n_flat;
+  int tile0[kMaxLayers + 1];  // first 32 x 32 tile of each layer (prefix sums)
+  const bf16_t* XT;
+  const bf16_t* AT;
+  const bf16_t* dZT;
+  const bf16_t* dYT;
+  float* slab;  // [slices][n_flat]
+};
+
+// Q K steps of one wave's 32 x 32 weight-gradient tile: every fragment is
+// requested before the first MFMA; with brow, the dZ fragments' column sums
+// too (the bias gradient)
+template <int Q>
+__device__ __forceinline__ void dw_steps(const bf16_t* __restrict__ A, const bf16_t* __restrict__ G, int B,
+                                         int m0, int n0, int k0, int fr, int fk, bool brow, f32x4 acc[2][2],
+                                         float bsum[2]) {
+  bf16x8 av[Q][2], bv[Q][2];
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      av[q][i] = load8(A + (size_t)(m0 + i * 16 + fr) * B + k0 + q * 32 + fk);
+      bv[q][i] = load8(G + (size_t)(n0 + i * 16 + fr) * B + k0 + q * 32 + fk);
+    }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[q][i], bv[q][j], acc[i][j], 0, 0, 0);
+    if (brow) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        float w = 0.f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w += (float)bv[q][j][e];
+        bsum[j] += w;
+      }
     }
   }
-  red[threadIdx.x] = l;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
+}
+
+// Weight gradients dW_l = A_{l-1}^T dZ_l of every layer: one wave per
+// 32 x 32 tile of one layer and one slice of the batch rows (rows of A^T
+// and dZ^T are k-contiguous, so both fragments are 16-byte loads).  The
+// waves of a layer's first tile row also sum their dZ fragments: the bias
+// gradient's partial over the slice.  Every word of a slice's slab (the
+// padding included) is written by exactly one wave.
+__global__ __launch_bounds__(256) void mlp_dweight(DwArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int L = a.net.depth + 1;
+  const int tiles = a.tile0[L];
+  const int s = gw / tiles;
+  if (s >= a.slices) return;
+  int t = gw - s * tiles;
+  int l = 0;
+  while (t >= a.tile0[l + 1]) ++l;
+  t -= a.tile0[l];
+  const int N = a.net.N[l];
+  const int m0 = (t / (N / 32)) * 32, n0 = (t % (N / 32)) * 32;
+  const bf16_t* A = l == 0 ? a.XT : a.AT + (size_t)(l - 1) * a.net.H * a.B;
+  const bf16_t* G = l == a.net.depth ? a.dYT : a.dZT + (size_t)l * a.net.H * a.B;
+  const int fr = lane & 15, fk = (lane >> 4) * 8, cr = (lane >> 4) * 4;
+  const int r0 = s * a.rows_per_slice;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum[2] = {0.f, 0.f};
+  const bool brow = (m0 == 0);
+  // Q K steps (32 Q rows) of fragments in flight at a time: four while a
+  // slice has 128 rows left, then one (a slice is a multiple of 32 rows)
+  const int r1 = r0 + a.rows_per_slice;
+  int k0 = r0;
+  for (; k0 + 128 <= r1; k0 += 128) dw_steps<4>(A, G, a.B, m0, n0, k0, fr, fk, brow, acc, bsum);
+  for (; k0 < r1; k0 += 32) dw_steps<1>(A, G, a.B, m0, n0, k0, fr, fk, brow, acc, bsum);
+  float* out = a.slab + (size_t)s * a.n_flat;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        out[a.net.w_off[l] + (size_t)(m0 + i * 16 + cr + r) * N + n0 + j * 16 + fr] = acc[i][j][r];
+  if (brow) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      // lanes fr, fr + 16, fr + 32, fr + 48 hold parts of the same column
+      float q = bsum[j];
+      q += __shfl_xor(q, 16);
+      q += __shfl_xor(q, 32);
+      if (lane < 16) out[a.net.b_off[l] + n0 + j * 16 + fr] = q;
+    }
   }
-  if (threadIdx.x == 0) loss_partial[blockIdx.x] = red[0];
 }
 
-// gather rows of the resident dataset into the batch buffers (features as
-// padded bf16, targets fp32)
-__global__ void gather_batch(const float* X, int n_in, const float* T, int n_out, const int* idx,
-                             int B, int ldx, bf16_t* Xb, float* Tb) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  const int r = idx[b];
-  for (int j = 0; j < ldx; ++j) Xb[(size_t)b * ldx + j] = f2bf(j < n_in ? X[(size_t)r * n_in + j] : 0.f);
-  for (int j = 0; j < n_out; ++j) Tb[(size_t)b * n_out + j] = T[(size_t)r * n_out + j];
+struct AdamArgs {
+  Net net;
+  float* w;
+  float* g;               // gradient output (when !adam)
+  float* m;
+  float* v;
+  const unsigned char* mask;
+  const float* slab;
+  int slices;
+  int n_flat;
+  int adam;
+  float lr_t, b1, b2, eps;
+  bf16_t* wt;
+  bf16_t* wb;
+};
+
+// refresh both BF16 copies of flat parameter i if it is a weight
+__device__ __forceinline__ void put_bf16(const Net& net, int i, float wi, bf16_t* wt, bf16_t* wb) {
+  for (int l = 0; l <= net.depth; ++l) {
+    const int rel = i - net.w_off[l];
+    if (rel >= 0 && rel < net.K[l] * net.N[l]) {
+      const int k = rel / net.N[l], n = rel - k * net.N[l];
+      const bf16_t h = f2bf(wi);
+      wb[net.wb_off[l] + rel] = h;
+      wt[net.wb_off[l] + n * net.K[l] + k] = h;
+      return;
+    }
+  }
 }
 
-__global__ void f32_to_bf16(const float* a, bf16_t* b, size_t n) {
+// Slab sums in a fixed order, then Adam (Kingma & Ba) with the bias
+// correction folded into lr_t; or, with adam == 0, the gradient itself.
+__global__ void mlp_adam(AdamArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_flat) return;
+  float gi = 0.f;
+  for (int s = 0; s < a.slices; ++s) gi += a.slab[(size_t)s * a.n_flat + i];
+  if (!a.adam) {
+    a.g[i] = gi;
+    return;
+  }
+  if (!a.mask[i]) return;
+  const float mi = a.b1 * a.m[i] + (1.f - a.b1) * gi;
+  const float vi = a.b2 * a.v[i] + (1.f - a.b2) * gi * gi;
+  a.m[i] = mi;
+  a.v[i] = vi;
+  const float wi = a.w[i] - a.lr_t * mi / (sqrtf(vi) + a.eps);
+  a.w[i] = wi;
+  put_bf16(a.net, i, wi, a.wt, a.wb);
+}
+
+// both BF16 copies from the FP32 weights (after set_params)
+__global__ void mlp_sync_bf16(Net net, const float* w, int n_flat, bf16_t* wt, bf16_t* wb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_flat) put_bf16(net, i, w[i], wt, wb);
+}
+
+__global__ void fill_zero(float* p, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = f2bf(a[i]);
-}
-
-// Adam (Kingma & Ba) with bias correction folded into the step size; keeps
-// the BF16 copy of the weights in sync.  mask[i] == 0 marks padding that
-// stays zero.
-__global__ void adam_step(float* w, bf16_t* wb, const float* g, float* m, float* v,
-                          const unsigned char* mask, size_t n, float lr_t, float b1, float b2,
-                          float eps) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !mask[i]) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr_t * mi / (sqrtf(vi) + eps);
-  w[i] = wi;
-  if (wb) wb[i] = f2bf(wi);
+  if (i < n) p[i] = 0.f;
 }
 
 }  // namespace mlpk
@@ -239,43 +450,34 @@ __global__ void adam_step(float* w, bf16_t* wb, const float* g, float* m, float*
 // ===========================================================================
 using namespace mlpk;
 
-struct MlpLayer {
-  int K, N;          // padded in/out
-  int k_log, n_log;  // logical in/out
-  size_t w_off, b_off;  // offsets into the flat fp32 parameter buffer
-  size_t wb_off;        // offset of the bf16 weight copy
-};
-
 struct ismc_mlp {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   std::string err;
   int n_in = 0, n_out = 0, hidden = 0, depth = 0, max_batch = 0;
-  int in_pad = 64, out_pad = 64;   // padded to the 64-wide GEMM tile
-  std::vector<MlpLayer> L;
-  size_t n_flat = 0, n_wb = 0;
+  int n_flat = 0, n_wb = 0;
   int64_t n_logical = 0;
-  int k_split = 16;
-  // device buffers
+  Net net{};
+  // parameters
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   unsigned char* mask = nullptr;
-  bf16_t* wb = nullptr;
-  bf16_t* X0 = nullptr;                 // [maxB][in_pad]
-  std::vector<float*> Z;                // per hidden layer [maxB][hidden]
-  std::vector<bf16_t*> A;               // per hidden layer [maxB][hidden]
-  float* Y = nullptr;                   // [maxB][out_pad]
-  bf16_t* dA = nullptr;                 // scratch dZ buffers
-  bf16_t* dB = nullptr;
-  float* slabs = nullptr;               // split-K partials
-  float* Tb = nullptr;                  // batch targets
-  float* lossp = nullptr;
-  float* lossd = nullptr;               // per-step loss partials of a training call
+  bf16_t *wt = nullptr, *wb = nullptr;
+  // host-batch staging and inference output (max_batch rows)
+  float* stage_x = nullptr;   // [max_batch][n_in]
+  float* stage_y = nullptr;   // [max_batch][n_out]
+  float* Y = nullptr;         // [max_batch][n_out]
+  float* lossp = nullptr;     // [max_batch / kRows]
+  // training buffers, allocated on first use for a batch size
+  int train_B = 0;
+  float* Zf = nullptr;
+  bf16_t *XT = nullptr, *AT = nullptr, *dZT = nullptr, *dYT = nullptr;
+  float* slab = nullptr;
+  float* lossd = nullptr;     // per-step loss partials of a training call
   size_t lossd_cap = 0;
-  int* idx = nullptr;
-  float* stage_x = nullptr;             // [maxB][n_in] upload staging
-  float* stage_y = nullptr;             // [maxB][n_out]
-  float* ds_x = nullptr;                // resident dataset
+  int* idx = nullptr;         // one epoch's dataset order
+  size_t idx_cap = 0;
+  float* ds_x = nullptr;      // resident dataset
   float* ds_y = nullptr;
   int64_t ds_n = 0;
   int64_t steps = 0;
@@ -306,140 +508,169 @@ static int dalloc(ismc_mlp* p, T** ptr, size_t count) {
   return 0;
 }
 
-template <bool TA, bool TB, int EPI>
-static int gemm(ismc_mlp* p, int M, int N, int K, const bf16_t* A, int lda, const bf16_t* B,
-                int ldb, float* C, int ldc, bf16_t* Cb, const float* bias, const float* Zp,
-                int k_split) {
-  if (M % TM || N % TN || K % (TK * k_split)) return mlp_err(p, ISMC_EINVAL, "gemm shape not tileable");
-  GemmArgs g{M, N, K, A, lda, B, ldb, C, ldc, Cb, bias, Zp, k_split};
-  dim3 grid(N / TN, M / TM, k_split);
-  gemm_kernel<TA, TB, EPI><<<grid, 256, 0, p->stream>>>(g);
-  MCHECK(p, hipGetLastError());
-  return 0;
+static void free_train(ismc_mlp* p) {
+  void* bufs[] = {p->Zf, p->XT, p->AT, p->dZT, p->dYT, p->slab};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  p->Zf = nullptr;
+  p->XT = p->AT = p->dZT = p->dYT = nullptr;
+  p->slab = nullptr;
+  p->train_B = 0;
 }
 
 static void mlp_free(ismc_mlp* p) {
-  void* bufs[] = {p->w, p->g, p->m, p->v, p->mask, p->wb, p->X0, p->Y, p->dA, p->dB,
-                  p->slabs, p->Tb, p->lossp, p->lossd, p->idx, p->ds_x, p->ds_y, p->stage_x, p->stage_y};
+  free_train(p);
+  void* bufs[] = {p->w, p->g, p->m, p->v, p->mask, p->wt, p->wb, p->stage_x, p->stage_y, p->Y,
+                  p->lossp, p->lossd, p->idx, p->ds_x, p->ds_y};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
-  for (auto z : p->Z)
-    if (z) (void)hipFree(z);
-  for (auto a : p->A)
-    if (a) (void)hipFree(a);
 }
 
-// forward on the X0 batch buffer (B rows); leaves Z/A/Y populated
-static int forward(ismc_mlp* p, int B) {
-  int rc;
-  const bf16_t* in = p->X0;
-  int ld_in = p->in_pad;
-  for (int l = 0; l < p->depth; ++l) {
-    const MlpLayer& Ly = p->L[l];
-    if ((rc = gemm<false, false, EPI_BIAS_SILU>(p, B, Ly.N, Ly.K, in, ld_in, p->wb + Ly.wb_off, Ly.N,
-                                                  p->Z[l], Ly.N, p->A[l], p->w + Ly.b_off, nullptr, 1)))
-      return rc;
-    in = p->A[l];
-    ld_in = Ly.N;
+// training buffers for batches of B rows
+static int ensure_train(ismc_mlp* p, int B) {
+  if (p->train_B >= B) return 0;
+  free_train(p);
+  const size_t H = p->hidden, D = p->depth;
+  int rc = dalloc(p, &p->Zf, D * H * B);
+  rc = rc ? rc : dalloc(p, &p->XT, (size_t)kInPad * B);
+  rc = rc ? rc : dalloc(p, &p->AT, D * H * B);
+  rc = rc ? rc : dalloc(p, &p->dZT, D * H * B);
+  rc = rc ? rc : dalloc(p, &p->dYT, (size_t)kOutPad * B);
+  rc = rc ? rc : dalloc(p, &p->slab, (size_t)kMaxSlices * p->n_flat);
+  if (rc) {
+    free_train(p);
+    return rc;
   }
-  const MlpLayer& Lo = p->L[p->depth];
-  return gemm<false, false, EPI_BIAS_F32>(p, B, Lo.N, Lo.K, in, ld_in, p->wb + Lo.wb_off, Lo.N, p->Y,
-                                          Lo.N, nullptr, p->w + Lo.b_off, nullptr, 1);
+  p->train_B = B;
+  return 0;
 }
 
-// loss + backward on the current batch (targets in Tb); gradients into g
-static int backward(ismc_mlp* p, int B, float* loss_host, float* loss_dev = nullptr) {
-  int rc;
-  const MlpLayer& Lo = p->L[p->depth];
-  const int nb = (B + 255) / 256;
-  mse_grad<<<nb, 256, 0, p->stream>>>(p->Y, Lo.N, p->Tb, p->n_out, B, p->dA, loss_dev ? loss_dev : p->lossp);
+template <int H>
+static void launch_fwd(ismc_mlp* p, const FwdArgs& a) {
+  mlp_forward<H><<<a.B / kRows, 64 * Waves<H>::value, 0, p->stream>>>(a);
+}
+template <int H>
+static void launch_bwd(ismc_mlp* p, const BwdArgs& a) {
+  mlp_backward<H><<<a.B / kRows, 64 * Waves<H>::value, 0, p->stream>>>(a);
+}
+
+static int run_forward(ismc_mlp* p, FwdArgs a) {
+  a.net = p->net;
+  switch (p->hidden) {
+    case 64: launch_fwd<64>(p, a); break;
+    case 128: launch_fwd<128>(p, a); break;
+    case 256: launch_fwd<256>(p, a); break;
+    default: launch_fwd<512>(p, a); break;
+  }
   MCHECK(p, hipGetLastError());
-  bf16_t* dcur = p->dA;
-  bf16_t* dnext = p->dB;
-  for (int l = p->depth; l >= 0; --l) {
-    const MlpLayer& Ly = p->L[l];
-    const bf16_t* act = l == 0 ? p->X0 : p->A[l - 1];
-    // dW = act^T dZ, split over the batch, partial slabs reduced in fixed order
-    int ks = p->k_split;
-    while (ks > 1 && B % (TK * ks)) ks >>= 1;
-    if ((rc = gemm<true, false, EPI_STORE_F32>(p, Ly.K, Ly.N, B, act, Ly.K, dcur, Ly.N, p->slabs, Ly.N,
-                                                 nullptr, nullptr, nullptr, ks)))
-      return rc;
-    const size_t nw = (size_t)Ly.K * Ly.N;
-    reduce_slabs<<<(unsigned)((nw + 255) / 256), 256, 0, p->stream>>>(p->slabs, ks, nw, p->g + Ly.w_off);
-    // bias gradient: column sums over kBiasSlabs row slabs (the split-K slabs
-    // above are consumed by now and hold the partials: kBiasSlabs x N floats
-    // fit in k_split x K x N), then a fixed-order sum
-    {
-      const int rows = (B + kBiasSlabs - 1) / kBiasSlabs;
-      const int nslab = (B + rows - 1) / rows;
-      bias_partial<<<dim3((unsigned)((Ly.N + 63) / 64), (unsigned)nslab), 256, 0, p->stream>>>(
-          dcur, B, Ly.N, rows, p->slabs);
-      reduce_slabs<<<(unsigned)((Ly.N + 255) / 256), 256, 0, p->stream>>>(p->slabs, nslab, (size_t)Ly.N,
-                                                                           p->g + Ly.b_off);
-    }
-    MCHECK(p, hipGetLastError());
-    if (l > 0) {
-      // dZ_prev = (dZ W^T) * silu'(Z_prev)
-      if ((rc = gemm<false, true, EPI_SILU_GRAD>(p, B, Ly.K, Ly.N, dcur, Ly.N, p->wb + Ly.wb_off, Ly.N,
-                                                  nullptr, Ly.K, dnext, nullptr, p->Z[l - 1], 1)))
-        return rc;
-      std::swap(dcur, dnext);
-    }
-  }
-  if (loss_host) {
-    std::vector<float> lp(nb);
-    MCHECK(p, hipMemcpyAsync(lp.data(), p->lossp, sizeof(float) * nb, hipMemcpyDeviceToHost, p->stream));
-    MCHECK(p, hipStreamSynchronize(p->stream));
-    double s = 0;
-    for (float x : lp) s += x;
-    *loss_host = (float)(s / ((double)B * p->n_out));
-  }
   return 0;
 }
 
-static int upload_batch(ismc_mlp* p, const float* x, const float* y, int B) {
-  // stage through persistent device buffers, rows 0..B-1 in order
-  std::vector<int> id(B);
-  for (int i = 0; i < B; ++i) id[i] = i;
-  float* tx = p->stage_x;
-  float* ty = y ? p->stage_y : nullptr;
-  hipError_t e = hipMemcpyAsync(tx, x, sizeof(float) * B * p->n_in, hipMemcpyHostToDevice, p->stream);
-  if (e == hipSuccess && y)
-    e = hipMemcpyAsync(ty, y, sizeof(float) * B * p->n_out, hipMemcpyHostToDevice, p->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(p->idx, id.data(), sizeof(int) * B, hipMemcpyHostToDevice, p->stream);
-  if (e == hipSuccess) {
-    gather_batch<<<(B + 255) / 256, 256, 0, p->stream>>>(tx, p->n_in, ty ? ty : tx, y ? p->n_out : 0,
-                                                         p->idx, B, p->in_pad, p->X0, p->Tb);
-    e = hipGetLastError();
+// forward with every training output (rows of X through idx, or in order;
+// targets T), backward, and the weight gradients into the slabs; returns
+// the number of slabs, or a negative status
+static int forward_backward(ismc_mlp* p, int B, const float* X, const float* T, const int* idx,
+                            float* loss_part) {
+  int rc;
+  if ((rc = ensure_train(p, B))) return rc;
+  FwdArgs f{};
+  f.B = B;
+  f.n_valid = B;
+  f.X = X;
+  f.T = T;
+  f.idx = idx;
+  f.Zf = p->Zf;
+  f.XT = p->XT;
+  f.AT = p->AT;
+  f.dYT = p->dYT;
+  f.loss_part = loss_part;
+  f.grad_scale = 2.0f / (float)(B * p->n_out);
+  if ((rc = run_forward(p, f))) return rc;
+  BwdArgs b{};
+  b.net = p->net;
+  b.B = B;
+  b.Zf = p->Zf;
+  b.dYT = p->dYT;
+  b.dZT = p->dZT;
+  switch (p->hidden) {
+    case 64: launch_bwd<64>(p, b); break;
+    case 128: launch_bwd<128>(p, b); break;
+    case 256: launch_bwd<256>(p, b); break;
+    default: launch_bwd<512>(p, b); break;
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-  if (e != hipSuccess) return mlp_err(p, ISMC_EDEVICE, "batch upload", e);
-  return 0;
+  MCHECK(p, hipGetLastError());
+  DwArgs d{};
+  d.net = p->net;
+  d.B = B;
+  int slices = kMaxSlices;
+  while (slices > 1 && B % (32 * slices)) slices >>= 1;
+  d.slices = slices;
+  d.rows_per_slice = B / slices;
+  d.n_flat = p->n_flat;
+  d.tile0[0] = 0;
+  for (int l = 0; l <= p->depth; ++l) d.tile0[l + 1] = d.tile0[l] + (p->net.K[l] / 32) * (p->net.N[l] / 32);
+  d.XT = p->XT;
+  d.AT = p->AT;
+  d.dZT = p->dZT;
+  d.dYT = p->dYT;
+  d.slab = p->slab;
+  const int waves = d.tile0[p->depth + 1] * slices;
+  mlp_dweight<<<(waves + 3) / 4, 256, 0, p->stream>>>(d);
+  MCHECK(p, hipGetLastError());
+  return slices;
 }
 
-// one Adam step on every layer from the gradients in p->g
-static int adam_all(ismc_mlp* p, float lr, float b1, float b2, float eps) {
-  p->steps++;
-  const float bc1 = 1.f - powf(b1, (float)p->steps), bc2 = 1.f - powf(b2, (float)p->steps);
-  const float lr_t = lr * sqrtf(bc2) / bc1;
-  for (const MlpLayer& Ly : p->L) {
-    const size_t nw = (size_t)Ly.K * Ly.N;
-    adam_step<<<(unsigned)((nw + 255) / 256), 256, 0, p->stream>>>(
-        p->w + Ly.w_off, p->wb + Ly.wb_off, p->g + Ly.w_off, p->m + Ly.w_off, p->v + Ly.w_off,
-        p->mask + Ly.w_off, nw, lr_t, b1, b2, eps);
-    adam_step<<<(unsigned)((Ly.N + 255) / 256), 256, 0, p->stream>>>(
-        p->w + Ly.b_off, nullptr, p->g + Ly.b_off, p->m + Ly.b_off, p->v + Ly.b_off, p->mask + Ly.b_off,
-        (size_t)Ly.N, lr_t, b1, b2, eps);
+// the slabs' sums into p->g (adam == false), or one Adam step
+static int reduce_step(ismc_mlp* p, int slices, bool adam, float lr, float b1, float b2, float eps) {
+  AdamArgs a{};
+  a.net = p->net;
+  a.w = p->w;
+  a.g = p->g;
+  a.m = p->m;
+  a.v = p->v;
+  a.mask = p->mask;
+  a.slab = p->slab;
+  a.slices = slices;
+  a.n_flat = p->n_flat;
+  a.adam = adam ? 1 : 0;
+  if (adam) {
+    p->steps++;
+    const float bc1 = 1.f - powf(b1, (float)p->steps), bc2 = 1.f - powf(b2, (float)p->steps);
+    a.lr_t = lr * sqrtf(bc2) / bc1;
   }
+  a.b1 = b1;
+  a.b2 = b2;
+  a.eps = eps;
+  a.wt = p->wt;
+  a.wb = p->wb;
+  mlp_adam<<<(p->n_flat + 255) / 256, 256, 0, p->stream>>>(a);
   MCHECK(p, hipGetLastError());
   return 0;
 }
 
 static int check_batch(ismc_mlp* p, int B) {
-  if (B <= 0 || B > p->max_batch || B % TM)
+  if (B <= 0 || B > p->max_batch || B % 64)
     return mlp_err(p, ISMC_EINVAL, "batch must be a positive multiple of 64 and <= max_batch");
+  return 0;
+}
+
+// host rows x[rows][n_in] (and y[rows][n_out]) into the staging buffers
+static int upload(ismc_mlp* p, const float* x, const float* y, int rows) {
+  MCHECK(p, hipMemcpyAsync(p->stage_x, x, sizeof(float) * rows * p->n_in, hipMemcpyHostToDevice, p->stream));
+  if (y)
+    MCHECK(p, hipMemcpyAsync(p->stage_y, y, sizeof(float) * rows * p->n_out, hipMemcpyHostToDevice, p->stream));
+  return 0;
+}
+
+// mean squared error of the batch from the per-block sums in p->lossp
+static int loss_of(ismc_mlp* p, int B, float* loss) {
+  const int nb = B / kRows;
+  std::vector<float> lp(nb);
+  MCHECK(p, hipMemcpyAsync(lp.data(), p->lossp, sizeof(float) * nb, hipMemcpyDeviceToHost, p->stream));
+  MCHECK(p, hipStreamSynchronize(p->stream));
+  double s = 0;
+  for (float x : lp) s += x;
+  *loss = (float)(s / ((double)B * p->n_out));
   return 0;
 }
 
@@ -449,8 +680,9 @@ int ismc_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t depth, int
                     int32_t max_batch, ismc_mlp** out) {
   if (!out) return ISMC_EINVAL;
   *out = nullptr;
-  if (n_in < 1 || n_in > 32 || n_out < 1 || n_out > 64 || hidden < 64 || hidden % 64 || depth < 1 ||
-      depth > 16 || max_batch < 64 || max_batch % 64)
+  if (n_in < 1 || n_in > kInPad || n_out < 1 || n_out > kOutPad ||
+      !(hidden == 64 || hidden == 128 || hidden == 256 || hidden == 512) || depth < 1 ||
+      depth > kMaxLayers - 1 || max_batch < 64 || max_batch % 64)
     return ISMC_EINVAL;
   ismc_mlp* p = new ismc_mlp();
   p->device = device;
@@ -461,24 +693,26 @@ int ismc_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t depth, int
   p->max_batch = max_batch;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&p->e0) != hipSuccess || hipEventCreate(&p->e1) != hipSuccess) {
-    delete p;
+    ismc_mlp_destroy(p);
     return ISMC_EDEVICE;
   }
-  size_t off = 0, woff = 0;
+  Net& net = p->net;
+  net.n_in = n_in;
+  net.n_out = n_out;
+  net.H = hidden;
+  net.depth = depth;
+  int off = 0, woff = 0;
   for (int l = 0; l <= depth; ++l) {
-    MlpLayer Ly;
-    Ly.K = l == 0 ? p->in_pad : hidden;
-    Ly.N = l == depth ? p->out_pad : hidden;
-    Ly.k_log = l == 0 ? n_in : hidden;
-    Ly.n_log = l == depth ? n_out : hidden;
-    Ly.w_off = off;
-    off += (size_t)Ly.K * Ly.N;
-    Ly.b_off = off;
-    off += Ly.N;
-    Ly.wb_off = woff;
-    woff += (size_t)Ly.K * Ly.N;
-    p->n_logical += (int64_t)Ly.k_log * Ly.n_log + Ly.n_log;
-    p->L.push_back(Ly);
+    net.K[l] = l == 0 ? kInPad : hidden;
+    net.N[l] = l == depth ? kOutPad : hidden;
+    const int k_log = l == 0 ? n_in : hidden, n_log = l == depth ? n_out : hidden;
+    net.w_off[l] = off;
+    off += net.K[l] * net.N[l];
+    net.b_off[l] = off;
+    off += net.N[l];
+    net.wb_off[l] = woff;
+    woff += net.K[l] * net.N[l];
+    p->n_logical += (int64_t)k_log * n_log + n_log;
   }
   p->n_flat = off;
   p->n_wb = woff;
@@ -489,44 +723,36 @@ int ismc_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t depth, int
   rc = rc ? rc : dalloc(p, &p->m, off);
   rc = rc ? rc : dalloc(p, &p->v, off);
   rc = rc ? rc : dalloc(p, &p->mask, off);
+  rc = rc ? rc : dalloc(p, &p->wt, woff);
   rc = rc ? rc : dalloc(p, &p->wb, woff);
-  rc = rc ? rc : dalloc(p, &p->X0, B * p->in_pad);
-  rc = rc ? rc : dalloc(p, &p->Y, B * p->out_pad);
-  rc = rc ? rc : dalloc(p, &p->dA, B * std::max(hidden, p->out_pad));
-  rc = rc ? rc : dalloc(p, &p->dB, B * std::max(hidden, p->out_pad));
-  rc = rc ? rc : dalloc(p, &p->slabs, (size_t)p->k_split * hidden * std::max(hidden, p->out_pad));
-  rc = rc ? rc : dalloc(p, &p->Tb, B * n_out);
-  rc = rc ? rc : dalloc(p, &p->lossp, (B + 255) / 256);
-  rc = rc ? rc : dalloc(p, &p->idx, B);
   rc = rc ? rc : dalloc(p, &p->stage_x, B * n_in);
   rc = rc ? rc : dalloc(p, &p->stage_y, B * n_out);
-  p->Z.assign(depth, nullptr);
-  p->A.assign(depth, nullptr);
-  for (int l = 0; l < depth && !rc; ++l) {
-    rc = dalloc(p, &p->Z[l], B * hidden);
-    rc = rc ? rc : dalloc(p, &p->A[l], B * hidden);
-  }
+  rc = rc ? rc : dalloc(p, &p->Y, B * n_out);
+  rc = rc ? rc : dalloc(p, &p->lossp, B / kRows);
   if (rc) {
-    mlp_free(p);
-    delete p;
+    ismc_mlp_destroy(p);
     return rc;
   }
+  net.w = p->w;
+  net.wt = p->wt;
+  net.wb = p->wb;
   // zero everything; the mask marks logical (non-padding) parameters
   std::vector<unsigned char> mask(off, 0);
-  for (const MlpLayer& Ly : p->L) {
-    for (int k = 0; k < Ly.k_log; ++k)
-      for (int n = 0; n < Ly.n_log; ++n) mask[Ly.w_off + (size_t)k * Ly.N + n] = 1;
-    for (int n = 0; n < Ly.n_log; ++n) mask[Ly.b_off + n] = 1;
+  for (int l = 0; l <= depth; ++l) {
+    const int k_log = l == 0 ? n_in : hidden, n_log = l == depth ? n_out : hidden;
+    for (int k = 0; k < k_log; ++k)
+      for (int n = 0; n < n_log; ++n) mask[net.w_off[l] + (size_t)k * net.N[l] + n] = 1;
+    for (int n = 0; n < n_log; ++n) mask[net.b_off[l] + n] = 1;
   }
   hipError_t e = hipMemcpy(p->mask, mask.data(), off, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(p->w, 0, off * 4);
-  if (e == hipSuccess) e = hipMemset(p->g, 0, off * 4);
-  if (e == hipSuccess) e = hipMemset(p->m, 0, off * 4);
-  if (e == hipSuccess) e = hipMemset(p->v, 0, off * 4);
-  if (e == hipSuccess) e = hipMemset(p->wb, 0, woff * 2);
+  if (e == hipSuccess) e = hipMemset(p->w, 0, (size_t)off * 4);
+  if (e == hipSuccess) e = hipMemset(p->g, 0, (size_t)off * 4);
+  if (e == hipSuccess) e = hipMemset(p->m, 0, (size_t)off * 4);
+  if (e == hipSuccess) e = hipMemset(p->v, 0, (size_t)off * 4);
+  if (e == hipSuccess) e = hipMemset(p->wt, 0, (size_t)woff * 2);
+  if (e == hipSuccess) e = hipMemset(p->wb, 0, (size_t)woff * 2);
   if (e != hipSuccess) {
-    mlp_free(p);
-    delete p;
+    ismc_mlp_destroy(p);
     return ISMC_EDEVICE;
   }
   *out = p;
@@ -536,11 +762,11 @@ int ismc_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t depth, int
 void ismc_mlp_destroy(ismc_mlp* p) {
   if (!p) return;
   (void)hipSetDevice(p->device);
-  (void)hipStreamSynchronize(p->stream);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
   mlp_free(p);
-  (void)hipEventDestroy(p->e0);
-  (void)hipEventDestroy(p->e1);
-  (void)hipStreamDestroy(p->stream);
+  if (p->e0) (void)hipEventDestroy(p->e0);
+  if (p->e1) (void)hipEventDestroy(p->e1);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
 
@@ -554,19 +780,19 @@ int ismc_mlp_set_params(ismc_mlp* p, const float* flat) {
   MCHECK(p, hipSetDevice(p->device));
   std::vector<float> h(p->n_flat, 0.f);
   size_t o = 0;
-  for (const MlpLayer& Ly : p->L) {
-    for (int k = 0; k < Ly.k_log; ++k)
-      for (int n = 0; n < Ly.n_log; ++n) h[Ly.w_off + (size_t)k * Ly.N + n] = flat[o++];
-    for (int n = 0; n < Ly.n_log; ++n) h[Ly.b_off + n] = flat[o++];
+  const Net& net = p->net;
+  for (int l = 0; l <= p->depth; ++l) {
+    const int k_log = l == 0 ? p->n_in : p->hidden, n_log = l == p->depth ? p->n_out : p->hidden;
+    for (int k = 0; k < k_log; ++k)
+      for (int n = 0; n < n_log; ++n) h[net.w_off[l] + (size_t)k * net.N[l] + n] = flat[o++];
+    for (int n = 0; n < n_log; ++n) h[net.b_off[l] + n] = flat[o++];
   }
-  MCHECK(p, hipMemcpyAsync(p->w, h.data(), p->n_flat * 4, hipMemcpyHostToDevice, p->stream));
-  for (const MlpLayer& Ly : p->L) {
-    const size_t n = (size_t)Ly.K * Ly.N;
-    f32_to_bf16<<<(unsigned)((n + 255) / 256), 256, 0, p->stream>>>(p->w + Ly.w_off, p->wb + Ly.wb_off, n);
-  }
+  MCHECK(p, hipMemcpyAsync(p->w, h.data(), (size_t)p->n_flat * 4, hipMemcpyHostToDevice, p->stream));
+  const unsigned nb = (unsigned)((p->n_flat + 255) / 256);
+  mlp_sync_bf16<<<nb, 256, 0, p->stream>>>(p->net, p->w, p->n_flat, p->wt, p->wb);
+  fill_zero<<<nb, 256, 0, p->stream>>>(p->m, (size_t)p->n_flat);
+  fill_zero<<<nb, 256, 0, p->stream>>>(p->v, (size_t)p->n_flat);
   MCHECK(p, hipGetLastError());
-  MCHECK(p, hipMemsetAsync(p->m, 0, p->n_flat * 4, p->stream));
-  MCHECK(p, hipMemsetAsync(p->v, 0, p->n_flat * 4, p->stream));
   MCHECK(p, hipStreamSynchronize(p->stream));
   p->steps = 0;
   return 0;
@@ -574,13 +800,15 @@ int ismc_mlp_set_params(ismc_mlp* p, const float* flat) {
 
 static int get_flat(ismc_mlp* p, const float* dev, float* flat) {
   std::vector<float> h(p->n_flat);
-  MCHECK(p, hipMemcpyAsync(h.data(), dev, p->n_flat * 4, hipMemcpyDeviceToHost, p->stream));
+  MCHECK(p, hipMemcpyAsync(h.data(), dev, (size_t)p->n_flat * 4, hipMemcpyDeviceToHost, p->stream));
   MCHECK(p, hipStreamSynchronize(p->stream));
   size_t o = 0;
-  for (const MlpLayer& Ly : p->L) {
-    for (int k = 0; k < Ly.k_log; ++k)
-      for (int n = 0; n < Ly.n_log; ++n) flat[o++] = h[Ly.w_off + (size_t)k * Ly.N + n];
-    for (int n = 0; n < Ly.n_log; ++n) flat[o++] = h[Ly.b_off + n];
+  const Net& net = p->net;
+  for (int l = 0; l <= p->depth; ++l) {
+    const int k_log = l == 0 ? p->n_in : p->hidden, n_log = l == p->depth ? p->n_out : p->hidden;
+    for (int k = 0; k < k_log; ++k)
+      for (int n = 0; n < n_log; ++n) flat[o++] = h[net.w_off[l] + (size_t)k * net.N[l] + n];
+    for (int n = 0; n < n_log; ++n) flat[o++] = h[net.b_off[l] + n];
   }
   return 0;
 }
@@ -595,23 +823,22 @@ int ismc_mlp_forward(ismc_mlp* p, const float* x, int32_t n, float* y) {
   if (!p || !x || !y || n < 0) return ISMC_EINVAL;
   MCHECK(p, hipSetDevice(p->device));
   int rc;
-  // host buffer for one batch's padded outputs, sized by this call's rows
-  const int32_t rows = std::min<int32_t>(p->max_batch, ((n + TM - 1) / TM) * TM);
-  std::vector<float> yb((size_t)rows * p->out_pad);
   for (int32_t s = 0; s < n; s += p->max_batch) {
     const int cnt = std::min<int32_t>(p->max_batch, n - s);
-    const int B = ((cnt + TM - 1) / TM) * TM;
-    std::vector<float> xb((size_t)B * p->n_in, 0.f);
-    memcpy(xb.data(), x + (size_t)s * p->n_in, sizeof(float) * cnt * p->n_in);
-    if ((rc = upload_batch(p, xb.data(), nullptr, B))) return rc;
+    const int B = ((cnt + kRows - 1) / kRows) * kRows;
+    if ((rc = upload(p, x + (size_t)s * p->n_in, nullptr, cnt))) return rc;
+    FwdArgs f{};
+    f.B = B;
+    f.n_valid = cnt;
+    f.X = p->stage_x;
+    f.Y = p->Y;
     MCHECK(p, hipEventRecord(p->e0, p->stream));
-    if ((rc = forward(p, B))) return rc;
+    if ((rc = run_forward(p, f))) return rc;
     MCHECK(p, hipEventRecord(p->e1, p->stream));
-    MCHECK(p, hipMemcpyAsync(yb.data(), p->Y, sizeof(float) * B * p->out_pad, hipMemcpyDeviceToHost, p->stream));
+    MCHECK(p, hipMemcpyAsync(y + (size_t)s * p->n_out, p->Y, sizeof(float) * cnt * p->n_out,
+                             hipMemcpyDeviceToHost, p->stream));
     MCHECK(p, hipStreamSynchronize(p->stream));
     MCHECK(p, hipEventElapsedTime(&p->last_ms, p->e0, p->e1));
-    for (int i = 0; i < cnt; ++i)
-      for (int j = 0; j < p->n_out; ++j) y[(size_t)(s + i) * p->n_out + j] = yb[(size_t)i * p->out_pad + j];
   }
   return 0;
 }
@@ -622,9 +849,11 @@ int ismc_mlp_gradients(ismc_mlp* p, const float* x, const float* y, int32_t B, f
   int rc;
   if ((rc = check_batch(p, B))) return rc;
   MCHECK(p, hipSetDevice(p->device));
-  if ((rc = upload_batch(p, x, y, B))) return rc;
-  if ((rc = forward(p, B))) return rc;
-  if ((rc = backward(p, B, loss))) return rc;
+  if ((rc = upload(p, x, y, B))) return rc;
+  const int slices = forward_backward(p, B, p->stage_x, p->stage_y, nullptr, p->lossp);
+  if (slices < 0) return slices;
+  if ((rc = reduce_step(p, slices, false, 0.f, 0.f, 0.f, 0.f))) return rc;
+  if (loss && (rc = loss_of(p, B, loss))) return rc;
   return get_flat(p, p->g, grads);
 }
 
@@ -633,28 +862,23 @@ int ismc_mlp_load_dataset(ismc_mlp* p, const float* x, const float* y, int64_t n
     return mlp_err(p, ISMC_EINVAL, "dataset needs max_batch .. 2^31-1 rows and non-null arrays");
   MCHECK(p, hipSetDevice(p->device));
   // the new buffers first: if one allocation fails the context keeps its
-  // previous dataset and index buffer intact
+  // previous dataset intact
   float* nx = nullptr;
   float* ny = nullptr;
-  int* nidx = nullptr;
   int rc = dalloc(p, &nx, (size_t)n * p->n_in);
   rc = rc ? rc : dalloc(p, &ny, (size_t)n * p->n_out);
-  rc = rc ? rc : dalloc(p, &nidx, (size_t)n);
   hipError_t e = hipSuccess;
   if (!rc) e = hipMemcpy(nx, x, sizeof(float) * n * p->n_in, hipMemcpyHostToDevice);
   if (!rc && e == hipSuccess) e = hipMemcpy(ny, y, sizeof(float) * n * p->n_out, hipMemcpyHostToDevice);
   if (rc || e != hipSuccess) {
     if (nx) (void)hipFree(nx);
     if (ny) (void)hipFree(ny);
-    if (nidx) (void)hipFree(nidx);
     return rc ? rc : mlp_err(p, ISMC_EDEVICE, "dataset upload", e);
   }
   if (p->ds_x) (void)hipFree(p->ds_x);
   if (p->ds_y) (void)hipFree(p->ds_y);
-  if (p->idx) (void)hipFree(p->idx);
   p->ds_x = nx;
   p->ds_y = ny;
-  p->idx = nidx;
   p->ds_n = n;
   return 0;
 }
@@ -669,14 +893,15 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
   int rc;
   if ((rc = check_batch(p, B))) return rc;
   if ((int64_t)steps_per_epoch * B > p->ds_n) return mlp_err(p, ISMC_EINVAL, "epoch larger than the dataset");
-  // every index must name a resident row: gather_batch reads ds_x / ds_y at
+  // every index must name a resident row: the forward reads ds_x / ds_y at
   // them, so one outside 0 .. ds_n-1 would read beyond the dataset
-  const size_t n_idx = (size_t)epochs * steps_per_epoch * B;
+  const size_t per_epoch = (size_t)steps_per_epoch * B;
+  const size_t n_idx = (size_t)epochs * per_epoch;
   for (size_t i = 0; i < n_idx; ++i)
     if (perm[i] < 0 || (int64_t)perm[i] >= p->ds_n)
       return mlp_err(p, ISMC_EINVAL, "perm holds an index outside the resident dataset");
   MCHECK(p, hipSetDevice(p->device));
-  const int nb = (B + 255) / 256;
+  const int nb = B / kRows;
   std::vector<float> lp((size_t)nb * steps_per_epoch);
   if (p->lossd_cap < lp.size()) {
     if (p->lossd) (void)hipFree(p->lossd);
@@ -685,23 +910,29 @@ int ismc_mlp_train(ismc_mlp* p, const int32_t* perm, int32_t epochs, int32_t ste
     if ((rc = dalloc(p, &p->lossd, lp.size()))) return rc;
     p->lossd_cap = lp.size();
   }
-  float* lossd = p->lossd;
+  if (p->idx_cap < per_epoch) {
+    if (p->idx) (void)hipFree(p->idx);
+    p->idx = nullptr;
+    p->idx_cap = 0;
+    if ((rc = dalloc(p, &p->idx, per_epoch))) return rc;
+    p->idx_cap = per_epoch;
+  }
+  if ((rc = ensure_train(p, B))) return rc;
   float ms_total = 0.f;
   for (int e = 0; e < epochs; ++e) {
-    MCHECK(p, hipMemcpyAsync(p->idx, perm + (size_t)e * steps_per_epoch * B,
-                             sizeof(int32_t) * (size_t)steps_per_epoch * B, hipMemcpyHostToDevice, p->stream));
+    MCHECK(p, hipMemcpyAsync(p->idx, perm + (size_t)e * per_epoch, sizeof(int32_t) * per_epoch,
+                             hipMemcpyHostToDevice, p->stream));
     MCHECK(p, hipEventRecord(p->e0, p->stream));
-    double esum = 0;
     for (int s = 0; s < steps_per_epoch; ++s) {
-      gather_batch<<<nb, 256, 0, p->stream>>>(p->ds_x, p->n_in, p->ds_y, p->n_out, p->idx + (size_t)s * B, B,
-                                              p->in_pad, p->X0, p->Tb);
-      MCHECK(p, hipGetLastError());
-      if ((rc = forward(p, B)) || (rc = backward(p, B, nullptr, lossd + (size_t)s * nb))) return rc;
-      if ((rc = adam_all(p, lr, b1, b2, eps))) return rc;
+      const int slices = forward_backward(p, B, p->ds_x, p->ds_y, p->idx + (size_t)s * B,
+                                          p->lossd + (size_t)s * nb);
+      if (slices < 0) return slices;
+      if ((rc = reduce_step(p, slices, true, lr, b1, b2, eps))) return rc;
     }
     MCHECK(p, hipEventRecord(p->e1, p->stream));
-    MCHECK(p, hipMemcpyAsync(lp.data(), lossd, sizeof(float) * lp.size(), hipMemcpyDeviceToHost, p->stream));
+    MCHECK(p, hipMemcpyAsync(lp.data(), p->lossd, sizeof(float) * lp.size(), hipMemcpyDeviceToHost, p->stream));
     MCHECK(p, hipStreamSynchronize(p->stream));
+    double esum = 0;
     for (float x : lp) esum += x / ((double)B * p->n_out);
     float ms = 0.f;
     MCHECK(p, hipEventElapsedTime(&ms, p->e0, p->e1));
@@ -721,9 +952,11 @@ int ismc_mlp_train_step(ismc_mlp* p, const float* x, const float* y, int32_t B, 
   int rc;
   if ((rc = check_batch(p, B))) return rc;
   MCHECK(p, hipSetDevice(p->device));
-  if ((rc = upload_batch(p, x, y, B))) return rc;
-  if ((rc = forward(p, B)) || (rc = backward(p, B, loss))) return rc;
-  if ((rc = adam_all(p, lr, b1, b2, eps))) return rc;
+  if ((rc = upload(p, x, y, B))) return rc;
+  const int slices = forward_backward(p, B, p->stage_x, p->stage_y, nullptr, p->lossp);
+  if (slices < 0) return slices;
+  if ((rc = reduce_step(p, slices, true, lr, b1, b2, eps))) return rc;
+  if (loss && (rc = loss_of(p, B, loss))) return rc;
   MCHECK(p, hipStreamSynchronize(p->stream));
   return 0;
 }

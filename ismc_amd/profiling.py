@@ -15,7 +15,8 @@ activity counters per packet.
     python -m ismc_amd.profiling --packets 268435456 --trace-packets 1073741824 \\
         --copy-bytes 1073741824 --trace KT --fetch F --write W --pmc A --pmc B > counters.json
 
-`--issue-rates` instead measures the device's VALU issue cost per wave64
+`--mlp-steps N` times N training steps of the surrogate network (batch
+4096) on synthetic rows.  `--issue-rates` instead measures the device's VALU issue cost per wave64
 instruction for single instruction kinds (ismc_issue_rate) and prints them
 as JSON: the per-cycle figures behind the transport kernel's roofline.
 The probes (ismc_issue_rate, ismc_copy_probe) live in libismc_diag.so
@@ -176,10 +177,45 @@ def issue_rates(device=0):
     return {"n_cu": n_cu, "simds": 4 * n_cu, "ops": out}
 
 
+def mlp_steps(steps=200, batch=4096, rows=65536, device=0, seed=0):
+    """Training steps of a fresh surrogate network (5 -> 256 x 4 -> 4) on
+    synthetic rows: `steps` Adam steps of `batch` rows in one ismc_mlp_train
+    call after a one-epoch warm-up; returns device ms per step and the
+    step's dense FLOP against the BF16 MFMA peak."""
+    import numpy as np
+
+    from . import surrogate as S
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (rows, 5)).astype(np.float32)
+    y = np.stack([np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2], np.tanh(x[:, 3] - x[:, 4]),
+                  x[:, 0] ** 2 - x[:, 1], np.cos(x[:, 2] + x[:, 4])], 1).astype(np.float32)
+    m = S.MLP(max_batch=batch, device=device)
+    try:
+        m.set_params(S.init_params(m.arch, seed))
+        m.load_dataset(x, y)
+        per_epoch = rows // batch
+        m.train(1, batch=batch, lr=1e-3, seed=seed)
+        epochs = max(1, steps // per_epoch)
+        losses, ms = m.train(epochs, batch=batch, lr=1e-3, seed=seed + 1)
+    finally:
+        m.close()
+    n = epochs * per_epoch
+    # logical network: 5 -> 256 x 4 -> 4; forward 2 flops per weight per row,
+    # backward twice that (dZ W^T and A^T dZ)
+    weights = 5 * 256 + 3 * 256 * 256 + 256 * 4
+    flop = 3 * 2 * weights * batch
+    ms_step = ms / n
+    return {"steps": n, "batch": batch, "ms_per_step": ms_step, "flop_per_step": flop,
+            "tflops": flop / (ms_step * 1e-3) / 1e12, "mfma_peak_tflops": 2500.0,
+            "loss_first_last": [float(losses[0]), float(losses[-1])]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--issue-rates", action="store_true",
                     help="measure VALU issue costs per instruction kind instead")
+    ap.add_argument("--mlp-steps", type=int, default=0,
+                    help="time this many surrogate training steps (batch 4096) instead")
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--packets", type=int, help="packets per launch of the counter runs")
     ap.add_argument("--trace-packets", type=int, help="packets per launch of the trace run (default --packets)")
@@ -192,6 +228,10 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.issue_rates:
         json.dump(issue_rates(a.device), sys.stdout, indent=1)
+        print()
+        return
+    if a.mlp_steps:
+        json.dump(mlp_steps(a.mlp_steps, device=a.device), sys.stdout, indent=1)
         print()
         return
     if a.packets is None:

@@ -25,6 +25,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="")
     ap.add_argument("--load", default="", help="skip training, load weights")
+    ap.add_argument("--data", default="", help="train on labels saved by --save-data instead of generating them")
+    ap.add_argument("--save-data", default="", help="save the generated inputs and labels (float32 .npz)")
     ap.add_argument("--inverse", type=int, default=0, help="measurements for the inverse benchmark")
     ap.add_argument("--plain", type=int, default=8)
     ap.add_argument("--refine-packets", type=int, default=100000)
@@ -43,13 +45,24 @@ def main(argv=None):
     if a.load:
         sur = surrogate.Surrogate.load(a.load)
     else:
-        log("generating %d samples x %d packets" % (a.samples, a.packets))
-        inp, tal, secs, packets = surrogate.generate_dataset(
-            a.samples, a.packets, seed=a.seed, engine=eng,
-            progress=lambda d, n, t: log("  labels %d/%d  %.1f s" % (d, n, t)))
-        result["dataset"] = {"samples": a.samples, "packets_per_sample": a.packets,
-                             "seconds": secs, "packets_per_s": packets / secs}
-        log("dataset: %.1f s (%.3g packets/s)" % (secs, packets / secs))
+        if a.data:
+            d = np.load(a.data, allow_pickle=False)
+            inp = {k: d[k].astype(np.float64) for k in ("mu_a", "mu_s", "g", "n", "t")}
+            tal = d["labels"].astype(np.float64)
+            a.packets = int(d["packets"])
+            result["dataset"] = {"samples": int(tal.shape[0]), "packets_per_sample": a.packets,
+                                 "loaded_from": a.data}
+        else:
+            log("generating %d samples x %d packets" % (a.samples, a.packets))
+            inp, tal, secs, packets = surrogate.generate_dataset(
+                a.samples, a.packets, seed=a.seed, engine=eng,
+                progress=lambda d, n, t: log("  labels %d/%d  %.1f s" % (d, n, t)))
+            result["dataset"] = {"samples": a.samples, "packets_per_sample": a.packets,
+                                 "seconds": secs, "packets_per_s": packets / secs}
+            log("dataset: %.1f s (%.3g packets/s)" % (secs, packets / secs))
+            if a.save_data:
+                np.savez(a.save_data, labels=tal.astype(np.float32), packets=np.int64(a.packets),
+                         **{k: v.astype(np.float32) for k, v in inp.items()})
         t0 = time.perf_counter()
         sur, info = surrogate.train_surrogate(inp, tal, epochs=a.epochs, batch=a.batch, lr=a.lr,
                                               seed=a.seed, lr_decay_every=max(1, a.epochs // 3))
