@@ -137,7 +137,7 @@ struct alignas(16) Iface {
 template <typename R>
 struct alignas(16) Layer {
   R zlo, zhi, n, step_k;     // step_k = ln2 / mu_t: s = -log2(u) * step_k
-  R absfrac, hg_a, hg_b, hg_c;  // mu_a/mu_t; 1+g^2; 1-g^2; 1-g
+  R absfrac, hg_a, hg_b, hg_c;  // mu_a/mu_t; 1+g^2; 1-g^2; 1-g (FP64) or 1+3g (FP32, see Draw)
   R hg_d, hg_e, mu_t;        // 2g; 1/(2g); mu_t
   int32_t kind;
   Iface<R> out[2];           // leaving towards lower z (0) / higher z (1)
@@ -224,6 +224,29 @@ __device__ __forceinline__ R u01(uint32_t x) {
   return (R)(2.0f - __builtin_bit_cast(float, m));
 }
 
+// A draw as the flight step keeps it.  FP32 keeps the float m = 2 - u =
+// 1 + (x >> 9) 2^-23 in [1, 2) (one v_alignbit_b32) and forms u = 2 - m
+// only where u itself is read: the Henyey-Greenstein and azimuth formulas
+// take m directly, hc + hd u = (1 + 3g) - 2g m (the layer's hg_c holds
+// 1 + 3g), 2u - 1 = 3 - 2m, and cos 2 pi u = cos 2 pi m, which saves the
+// subtraction of two of each event's three uniforms.  FP64 keeps u and the
+// formulas written with it, operation for operation as the oracle has them.
+template <typename R> struct Draw;
+template <> struct Draw<float> {
+  static __device__ __forceinline__ float keep(uint32_t x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_alignbit(0x7Fu, x, 9u));
+  }
+  static __device__ __forceinline__ float u(float m) { return 2.0f - m; }
+  static __device__ __forceinline__ float hg_cos(int kind, float ha, float hb, float hc3, float hd, float he,
+                                                 float m) {
+    const float q = hb * DM<float>::rcp(__builtin_fmaf(-hd, m, hc3));
+    const float c = (ha - q * q) * he;
+    return kind == KIND_ISO ? __builtin_fmaf(-2.0f, m, 3.0f) : c;
+  }
+  static __device__ __forceinline__ float cos2pi(float m) { return DM<float>::cos2pi(m); }
+  static constexpr bool kHc3 = true;   // hg_c holds 1 + 3g
+};
+
 // Unpolarised Fresnel reflectance for light going from index ni to nt at
 // direction cosine mu_z (fresnel_snell, CUDAISMLMC.h:79-96), written with
 // cosines instead of angles: rs = (ni ci - nt ct)/(ni ci + nt ct),
@@ -267,11 +290,23 @@ __device__ __forceinline__ R hg_cos(int kind, R ha, R hb, R hc, R hd, R he, R u)
   return kind == KIND_ISO ? (R)2 * u - (R)1 : c;
 }
 
-// New mu_z after a scattering by theta with azimuth 2 pi u_phi
-// (CUDAISMLMC.h:287-308, z component).
+template <> struct Draw<double> {
+  static __device__ __forceinline__ double keep(uint32_t x) { return u01<double>(x); }
+  static __device__ __forceinline__ double u(double v) { return v; }
+  static __device__ __forceinline__ double hg_cos(int kind, double ha, double hb, double hc, double hd,
+                                                  double he, double u) {
+    return ::ismc::hg_cos<double>(kind, ha, hb, hc, hd, he, u);
+  }
+  static __device__ __forceinline__ double cos2pi(double u) { return DM<double>::cos2pi(u); }
+  static constexpr bool kHc3 = false;  // hg_c holds 1 - g
+};
+
+// New mu_z after a scattering by theta with azimuth 2 pi u_phi, the
+// azimuth given as the draw the flight step keeps (CUDAISMLMC.h:287-308,
+// z component).
 template <typename R>
 __device__ __forceinline__ R spin_z(R mu_z, R cos_t, R u_phi) {
-  const R cphi = DM<R>::cos2pi(u_phi);
+  const R cphi = Draw<R>::cos2pi(u_phi);
   const R st_sz = DM<R>::sqrt(fmax((R)0, ((R)1 - cos_t * cos_t) * ((R)1 - mu_z * mu_z)));
   const R general = mu_z * cos_t - st_sz * cphi;
   // near-vertical: mu_z' = sign(mu_z) * cos_t (cos_t keeps its own sign)
@@ -401,8 +436,8 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         if (DM<R>::abs(muz) >= P.cos_spec[side]) {
           term = side ? 4 : 2;
           phase = PH_END;
-        } else if (su1 < P.p_return[side]) {
-          muz = side ? -su2 : su2;
+        } else if (Draw<R>::u(su1) < P.p_return[side]) {
+          muz = side ? -Draw<R>::u(su2) : Draw<R>::u(su2);
           phase = PH_ENTER;
           e_rei += 1;
           if (ev > kMaxEventsPerPacket) { phase = PH_END; aborted = true; }
@@ -589,8 +624,8 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     // lane waiting for that read runs no event meanwhile.
     if (run_flight || run_enter) {
       bu0 = u01<R>(rw[0]);
-      su1 = u01<R>(rw[1]);
-      su2 = u01<R>(rw[2]);
+      su1 = Draw<R>::keep(rw[1]);
+      su2 = Draw<R>::keep(rw[2]);
     }
     const R u0 = bu0, u1 = su1, u2 = su2, u3 = u01<R>(rw[3]);
 
@@ -612,7 +647,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         const bool survive = !P.cuda_roulette && (u3 <= P.roul_p);
         w_new = survive ? w_new * P.roul_m : (R)0;
       }
-      const R ct = hg_cos<R>(L_kind, L_ha, L_hb, L_hc, L_hd, L_he, u1);
+      const R ct = Draw<R>::hg_cos(L_kind, L_ha, L_hb, L_hc, L_hd, L_he, u1);
       const R muz_new = spin_z<R>(muz, ct, u2);
       const R s_new = DM<R>::nlog2(u0) * L_step;
       const bool live = w_new > (R)0;
@@ -643,7 +678,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         L_kind = NL.kind;
         muz = DM<R>::csign(glass_entry_cos<R>(ct, P.ent_cmin[side]), muz);
         z = side ? P.t_tot : (R)0;
-        s = L_kind == KIND_GLASS ? (R)0 : DM<R>::nlog2(u1) * L_step;
+        s = L_kind == KIND_GLASS ? (R)0 : DM<R>::nlog2(Draw<R>::u(u1)) * L_step;
         phase = PH_FLIGHT;
       }
     }
@@ -903,7 +938,7 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
       Ly.absfrac = (R)(cfg->mu_a[i] / mu_t);
       Ly.hg_a = (R)(1 + g * g);
       Ly.hg_b = (R)(1 - g * g);
-      Ly.hg_c = (R)(1 - g);
+      Ly.hg_c = Draw<R>::kHc3 ? (R)(1 + 3 * g) : (R)(1 - g);
       Ly.hg_d = (R)(2 * g);
       Ly.hg_e = g == 0.0 ? (R)0 : (R)(1 / (2 * g));
       Ly.mu_t = (R)mu_t;
