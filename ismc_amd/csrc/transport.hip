@@ -92,9 +92,10 @@ constexpr uint64_t kMaxPacketsPerPoint = 1ull << 31;
 // steps +4 % over 1, with 4 a further +0.1 % / +0.3 % / +3.4 % (6: no gain).
 template <typename R> struct FlightSteps { static constexpr int value = 4; };
 template <> struct FlightSteps<float> { static constexpr int value = 6; };
-// waves per SIMD the FP32 tally kernel is compiled for (66 VGPRs: no
-// spills); the FP64 kernel needs about twice the registers and the
-// per-packet output variant a few more, both are left to the compiler
+// waves per SIMD the FP32 tally kernel is compiled for at least (it takes
+// 64 VGPRs, no spills, so 8 fit; 7 and 8 measured the same); the FP64
+// kernel needs about twice the registers and the per-packet output
+// variant a few more, both are left to the compiler
 template <typename R, bool kPerPacket> struct MinWaves { static constexpr int value = 1; };
 template <> struct MinWaves<float, false> { static constexpr int value = 7; };
 constexpr int kSlotWords = 16;  // per point: sum[5], sumsq[5], events[5], aborted
@@ -303,12 +304,22 @@ template <> struct Draw<double> {
 
 // New mu_z after a scattering by theta with azimuth 2 pi u_phi, the
 // azimuth given as the draw the flight step keeps (CUDAISMLMC.h:287-308,
-// z component).
+// z component).  The reference takes mu_z' = sign(mu_z) cos_t where
+// |mu_z| > 0.99999, because its x and y components divide by
+// sqrt(1 - mu_z^2).  The z component alone has no such division: the
+// general form is exact there too (at |mu_z| = 1 it is that same
+// sign(mu_z) cos_t) and differs from the shortcut by at most
+// sqrt(2e-5) sin(theta) |cos(phi)|, zero on average over phi.  The FP32
+// kernel uses the general form everywhere (three instructions fewer per
+// interaction, 2.5 % faster; slab tallies unchanged to six digits at
+// 2^29 packets); the FP64 kernel keeps the shortcut, operation for
+// operation as the oracle has it.
 template <typename R>
 __device__ __forceinline__ R spin_z(R mu_z, R cos_t, R u_phi) {
   const R cphi = Draw<R>::cos2pi(u_phi);
   const R st_sz = DM<R>::sqrt(fmax((R)0, ((R)1 - cos_t * cos_t) * ((R)1 - mu_z * mu_z)));
   const R general = mu_z * cos_t - st_sz * cphi;
+  if constexpr (sizeof(R) == 4) return general;
   // near-vertical: mu_z' = sign(mu_z) * cos_t (cos_t keeps its own sign)
   return DM<R>::abs(mu_z) > (R)0.99999 ? (mu_z > (R)0 ? cos_t : -cos_t) : general;
 }
@@ -382,17 +393,23 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
   // lane state
   int phase = PH_NEED, side = 0, layer = 0, set = 0, term = 0;
   bool aborted = false;
+  // A is the packet's starting weight plus the weight Russian roulette has
+  // added or removed: the weight absorbed so far is A - w (every other
+  // change of w is an absorption), so an interaction updates w alone
   R w = 0, z = 0, muz = 1, s = 0, A = 0;
   R su1 = 0, su2 = 0;  // return draw and new |mu_z| of a pending sphere decision
   R bu0 = 0;           // Fresnel draw of a pending interface hit
   uint64_t pid = 0, outidx = 0;
   uint32_t ev = 0, stream = 0;
   PhiloxFixed pf{0u, 0u, 0u, 0u, 0u};  // the packet's event-independent Philox products
-  uint32_t e_int = 0, e_bnd = 0, e_inc = 0, e_sph = 0, e_rei = 0;
+  // event counts of the packet; its interactions are ev - e_bnd - e_inc
+  // (every event is an interaction, an interface hit or an entrance)
+  uint32_t e_bnd = 0, e_inc = 0, e_sph = 0, e_rei = 0;
   // current layer cache
   R L_zlo = 0, L_zhi = 0, L_step = 0, L_abs = 0, L_ha = 0, L_hb = 0, L_hc = 0, L_hd = 0,
     L_he = 0;
   int L_kind = KIND_GLASS;
+  R L_thr = 0;  // the point's roulette threshold
 
   for (;;) {
     // Lanes in flight run their next event every flight step.  A flight
@@ -450,9 +467,9 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       const bool ending = (phase == PH_END);
       if (ending) {
         unsigned long long* acc = set ? acc1 : acc0;
+        A = aborted ? (R)0 : A - w;  // the absorbed weight
         if (aborted) {
           atomicAdd(&acc[15], 1ull);
-          A = (R)0;
           term = 0;
         } else {
           atomicAdd(&acc[0], DM<R>::fx(A));
@@ -462,7 +479,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
             atomicAdd(&acc[5 + term], DM<R>::fx(w * w));
           }
         }
-        atomicAdd(&acc[10], (unsigned long long)e_int);
+        atomicAdd(&acc[10], (unsigned long long)(ev - e_bnd - e_inc));
         atomicAdd(&acc[11], (unsigned long long)e_bnd);
         atomicAdd(&acc[12], (unsigned long long)e_inc);
         atomicAdd(&acc[13], (unsigned long long)e_sph);
@@ -476,7 +493,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
           o[4] = term == 4 ? (double)w : 0.0;
           o[5] = (double)ev;
         }
-        e_int = e_bnd = e_inc = e_sph = e_rei = 0;
+        e_bnd = e_inc = e_sph = e_rei = 0;
         term = 0;
         aborted = false;
         phase = PH_NEED;
@@ -544,7 +561,8 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
           w = (R)1;
           muz = pts[set].mu0_z;
           side = 0;
-          A = (R)0;
+          A = (R)1;
+          L_thr = pts[set].roul_thr;
           ev = 0;
           phase = PH_ENTER;
         }
@@ -566,7 +584,10 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     if (step > 0) {
       const uint64_t f = __ballot(phase == PH_FLIGHT);
       const uint64_t b = __ballot(phase == PH_BOUND);
-      bpass = b != 0 && (f == 0 || (uint32_t)__popcll(b) >= bound_thr);
+      // b != 0 && (f == 0 || |b| >= bound_thr), with bound_thr >= 1 (as
+      // 32-bit popcounts: scalar instructions only)
+      bpass = (uint32_t)(__builtin_popcount((uint32_t)b) + __builtin_popcount((uint32_t)(b >> 32))) >=
+              (f == 0 ? 1u : bound_thr);
     }
     // ---------------- interface pass: the hits recorded since the last one
     // (CUDAISMLMC.h:180-257), with the draws of the hit's own event
@@ -633,19 +654,25 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       const R zb = muz > (R)0 ? L_zhi : L_zlo;
       const R dz = zb - z;
       const bool glass = (L_kind == KIND_GLASS);
-      // bitwise, not short-circuit: no branches around the comparisons
-      const bool hit = glass | ((muz != (R)0) & (s * DM<R>::abs(muz) >= DM<R>::abs(dz)));
+      // The reference hits when s >= d = dz / mu_z, with d infinite for
+      // mu_z = 0 (MLMC.py:100-114).  Here s |mu_z| > |dz|: the same test
+      // except where the two are exactly equal (an interaction that ends on
+      // the interface, whose next event is then the hit), and mu_z = 0 never
+      // hits without a compare of its own (1.2 % faster).  Bitwise, not
+      // short-circuit: no branches around the comparisons.
+      const bool hit = glass | (s * DM<R>::abs(muz) > DM<R>::abs(dz));
       // A hit reaches the interface (its draws wait in bu0, su1, su2);
       // otherwise hop, drop, roulette, spin (CUDAISMLMC.h:259-309).  The
       // interaction's new state is computed for every lane in flight (the
       // wave runs this code whenever one of its lanes interacts) and taken
       // where the lane interacted, which saves the copies that merging two
       // branches costs.
-      const R dW = w * L_abs;
-      R w_new = w - dW;
-      if (!hit && w_new <= P.roul_thr) {
+      R w_new = w - w * L_abs;
+      if (!hit && w_new <= L_thr) {
         const bool survive = !P.cuda_roulette && (u3 <= P.roul_p);
-        w_new = survive ? w_new * P.roul_m : (R)0;
+        const R w_r = survive ? w_new * P.roul_m : (R)0;
+        A += w_r - w_new;
+        w_new = w_r;
       }
       const R ct = Draw<R>::hg_cos(L_kind, L_ha, L_hb, L_hc, L_hd, L_he, u1);
       const R muz_new = spin_z<R>(muz, ct, u2);
@@ -653,15 +680,14 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       const bool live = w_new > (R)0;
       if (!hit) {
         z += s * muz;
-        A += dW;
         w = w_new;
-        e_int += 1;
       }
       if (!hit && live) {
         muz = muz_new;
         s = s_new;
       }
       phase = hit ? PH_BOUND : (live ? PH_FLIGHT : PH_END);
+      ev += 1;
     } else if (run_enter) {
       // (re-)entrance through the outer face (CUDAISMLMC.h:367-369, 502-503)
       e_inc += 1;
@@ -681,11 +707,11 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         s = L_kind == KIND_GLASS ? (R)0 : DM<R>::nlog2(Draw<R>::u(u1)) * L_step;
         phase = PH_FLIGHT;
       }
+      ev += 1;
     }
     // a packet that goes on flying past the event limit is dropped at the
     // top of the next iteration; one waiting at an interface or for its
     // sphere decision is checked when it is known whether it continues
-    if (run_flight || run_enter) ev += 1;
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

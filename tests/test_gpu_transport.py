@@ -87,6 +87,25 @@ def test_eval_physics_matches_oracle(engine):
     assert np.max(np.abs(out[:, 0] - expect)) < 1e-12
 
 
+def test_fp32_spin_is_the_exact_rotation(engine):
+    # The FP32 kernel takes the general z component for every mu_z (no
+    # near-vertical shortcut): within FP32 rounding (hardware cos and sqrt)
+    # of the exact rotation, equal to sign(mu_z) cos_t at |mu_z| = 1, and
+    # within sqrt(2e-5) of the shortcut that the FP64 kernel keeps for
+    # 0.99999 < |mu_z| < 1.
+    rng = np.random.default_rng(2)
+    mz = np.concatenate([[-1.0, 1.0, 0.999995, -0.999999, 0.99999, 0.5, 0.0, -0.3], rng.uniform(-1, 1, 40)])
+    sp = np.array([(m, g, a, b) for m in mz for g in (0.0, 0.9) for a, b in rng.uniform(1e-6, 1, (10, 2))])
+    out = engine.eval_physics(5, sp, fp64=False)
+    ct = out[:, 1]
+    exact = sp[:, 0] * ct - np.sqrt(np.maximum(0, (1 - ct ** 2) * (1 - sp[:, 0] ** 2))) * np.cos(2 * np.pi * sp[:, 3])
+    assert np.max(np.abs(out[:, 0] - exact)) < 1e-5   # v_cos_f32 / v_sqrt_f32 rounding
+    vert = np.abs(sp[:, 0]) == 1.0
+    assert np.array_equal(out[vert, 0], (np.sign(sp[:, 0]) * ct)[vert].astype(np.float32))
+    near = np.abs(sp[:, 0]) > 0.99999
+    assert np.max(np.abs(out[near, 0] - np.sign(sp[near, 0]) * ct[near])) <= np.sqrt(2e-5) + 1e-5
+
+
 @pytest.mark.parametrize("name,cfg", CASES, ids=[c[0] for c in CASES])
 def test_fp64_kernel_matches_oracle_per_packet(engine, name, cfg):
     n = 4000
