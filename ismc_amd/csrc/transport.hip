@@ -128,7 +128,8 @@ struct alignas(16) Iface {
   R cmin;  // entering a glass slide from a less dense layer: the least |mu_z|
            // inside it (glass_entry_cos); 0 otherwise
   R mut;   // mu_t here / mu_t there: rescales the remaining step s when both
-           // layers scatter
+           // layers scatter; -1 where the packet leaves the stack (the
+           // interface pass reads "outside" from its sign, no layer count)
 };
 
 // Constants of one layer, derived on the host in FP64.  n and mu_t are not
@@ -592,25 +593,30 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     // ---------------- interface pass: the hits recorded since the last one
     // (CUDAISMLMC.h:180-257), with the draws of the hit's own event
     if (bpass && phase == PH_BOUND) {
-      const R zb = muz > (R)0 ? L_zhi : L_zlo;
+      const bool up = muz > (R)0;  // the direction the packet meets the interface in
+      const R zb = up ? L_zhi : L_zlo;
       const R dz = zb - z;
       const bool glass = (L_kind == KIND_GLASS);
       const R d = dz * DM<R>::rcp(muz);
       z = zb;
       s = glass ? (R)0 : s - d;
-      const int nxt = muz > (R)0 ? layer + 1 : layer - 1;
-      const bool outside = (nxt < 0) || (nxt >= P.n_layers);
-      const Iface<R> f = P.layer[layer].out[muz > (R)0 ? 1 : 0];
+      const int nxt = up ? layer + 1 : layer - 1;
+      const Iface<R> f = P.layer[layer].out[up ? 1 : 0];
+      const bool outside = f.mut < (R)0;  // nxt < 0 or nxt >= n_layers, from the host
       R ct;
       const R Rf = fresnel<R>(f.eta, f.qq, muz, ct);
       e_bnd += 1;
       phase = PH_FLIGHT;
-      if (bu0 < Rf) {
+      // reflect when u <= R: the reference draws u in [0, 1) and reflects
+      // when u < R (MLMC.py:122-144); this u is in (0, 1], and with u <= R
+      // the reflection probability is R on the draws' grid and total
+      // internal reflection (R = 1) reflects always
+      if (bu0 <= Rf) {
         muz = -muz;
       } else {
         muz = DM<R>::csign(ct, muz);
         if (outside) {
-          side = nxt < 0 ? 0 : 1;
+          side = up ? 1 : 0;
           phase = PH_SPHERE;  // su1, su2 hold the hit's return draw and new |mu_z|
         } else {
           // f.cmin is 0 unless the next layer is a denser slide, and f.mut
@@ -625,8 +631,9 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         }
       }
       // a packet still running after kMaxEventsPerPacket events is dropped
-      // (counted, not tallied)
-      if (phase == PH_FLIGHT && ev > kMaxEventsPerPacket) {
+      // (counted, not tallied); only a wave that was near the limit at the
+      // top of this iteration can hold such a lane
+      if (near_limit && phase == PH_FLIGHT && ev > kMaxEventsPerPacket) {
         phase = PH_END;
         aborted = true;
       }
@@ -979,6 +986,7 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
       P->layer[i].out[dir] = make_iface<R>(cfg->n[i], inside ? cfg->n[j] : namb, turbid, cfg->mu_t[i],
                                            inside && cfg->mu_s[j] != 0.0, inside && cfg->mu_s[j] == 0.0,
                                            inside ? cfg->mu_t[j] : 1.0);
+      if (!inside) P->layer[i].out[dir].mut = (R)-1;
     }
   }
   P->t_tot = (R)z;
