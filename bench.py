@@ -198,10 +198,14 @@ def surrogate_bench(eng, n_samples=16384, packets=100_000):
     1.33-1.50, t 0.05-0.5 cm, log-uniform / uniform; 1e5 packets per label,
     a seed the training never used), the label generation's packets/s, and
     the device time of one training step (batch 4096) of a fresh network on
-    those labels."""
+    those labels; the label launches' kernel time and slot roofline (their
+    histories are long: mu_s t reaches 500)."""
     from ismc_amd import surrogate as S
     sur = S.Surrogate.load(S.packaged_weights())
     rep = S.holdout_report(sur, n_samples=n_samples, packets=packets, engine=eng)
+    ev = rep.pop("label_events")
+    npk = rep["label_packets"]
+    rep["label_roofline"] = roofline_of(slots_per_packet(ev, npk), npk, rep["label_kernel_seconds"], eng, ev)
     inp = rep.pop("inputs")
     lab = rep.pop("labels")
     rep.pop("label_se")

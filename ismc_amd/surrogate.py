@@ -64,14 +64,15 @@ def sample_config(mu_a, mu_s, g, n, t):
 
 
 def generate_dataset(n_samples, packets, seed=0, engine: Optional[Engine] = None,
-                     points_per_launch=16384, progress=None, autotune=True, return_se=False):
+                     points_per_launch=16384, progress=None, autotune=True, return_se=False, stats=None):
     """MC labels for n_samples inputs: one batched launch per
     points_per_launch samples (one work-queue chunk per sample unless
     `autotune` picks the launch shape; labels are identical either way).
     Returns (inputs dict, tallies [n, 4] in OUTPUTS order, seconds, packets),
     with return_se the labels' MC standard errors [n, 4] before the seconds
     (R_tot's from E[R_d^2] + E[R_s^2]: a packet leaves its weight in one
-    exit tally)."""
+    exit tally).  A `stats` dict, when given, receives the launches' kernel
+    time ("kernel_ms") and summed event counts ("events")."""
     eng = engine or default_engine()
     inp = sample_inputs(n_samples, seed)
     tal = np.zeros((n_samples, 4))
@@ -84,6 +85,12 @@ def generate_dataset(n_samples, packets, seed=0, engine: Optional[Engine] = None
         res = eng.simulate_batch(cfgs, packets, seed=seed + 7, streams=np.arange(s, e, dtype=np.uint32),
                                  chunk_packets=0 if autotune else min(int(packets), 1 << 20),
                                  autotune=autotune)
+        if stats is not None:
+            stats["kernel_ms"] = stats.get("kernel_ms", 0.0) + eng.last_launch()["kernel_ms"]
+            ev = stats.setdefault("events", {})
+            for r in res:
+                for name, v in r.events.items():
+                    ev[name] = ev.get(name, 0) + v
         for k, r in enumerate(res):
             m = r.mean
             tal[s + k] = (m["R_d"], m["T_d"], m["T_u"], m["R_d"] + m["R_s"])
@@ -326,7 +333,9 @@ def holdout_report(sur, n_samples=16384, packets=100_000, seed=HOLDOUT_SEED, eng
     sqrt(max(RMSE^2 - mean SE^2, 0)).  Also the label generation's
     packets/s."""
     eng = engine or default_engine()
-    inp, tal, se, secs, npk = generate_dataset(n_samples, packets, seed=seed, engine=eng, return_se=True)
+    st = {}
+    inp, tal, se, secs, npk = generate_dataset(n_samples, packets, seed=seed, engine=eng, return_se=True,
+                                               stats=st)
     p = sur.predict(inp["mu_a"], inp["mu_s"], inp["g"], inp["n"], inp["t"])
     out = {}
     for i, k in enumerate(OUTPUTS):
@@ -340,6 +349,8 @@ def holdout_report(sur, n_samples=16384, packets=100_000, seed=HOLDOUT_SEED, eng
                   "network_rmse_noise_removed": max(rmse * rmse - msq, 0.0) ** 0.5}
     return {"samples": n_samples, "packets_per_label": int(packets), "seed": seed,
             "label_packets": int(npk), "label_seconds": secs, "label_packets_per_s": npk / secs,
+            "label_kernel_seconds": st["kernel_ms"] / 1e3,
+            "label_kernel_packets_per_s": npk / (st["kernel_ms"] / 1e3), "label_events": st["events"],
             "outputs": out, "inputs": inp, "labels": tal, "label_se": se}
 
 

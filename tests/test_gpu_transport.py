@@ -124,7 +124,10 @@ def test_fp32_kernel_tracks_oracle(engine, name, cfg):
     o = O.simulate(spec_of(cfg), n, seed=5, per_packet=True, flags=O.FP32)["per_packet"]
     same = np.all(np.abs(g[:, :5] - o[:, :5]) < 1e-4, axis=1)
     print(name, "FP32 packets within 1e-4 of the FP32 oracle: %.4f" % same.mean())
-    assert same.mean() >= 0.95
+    # measured 0.9990-1.0000 on these cases: a packet parts from the oracle's
+    # history only where FP32 rounding (or the exact spin against the
+    # oracle's near-vertical shortcut) flips one of its decisions
+    assert same.mean() >= 0.99
 
 
 @pytest.mark.parametrize("case", golden_cases())
