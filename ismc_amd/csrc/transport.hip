@@ -506,6 +506,10 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
 
       // ---- refill idle lanes from the wave's window
       uint64_t need = __ballot(phase == PH_NEED);
+      // The loop only hands out packet ids (a chunk switch can take it round
+      // twice); the lanes it served are set up once after it, so that their
+      // state is not carried round the loop (register copies at every pass)
+      uint64_t fresh = 0;  // lanes given a packet in this pass
       while (need) {
         if (win_next == win_end) {
           if (exhausted) {
@@ -553,23 +557,27 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         const uint32_t nneed = (uint32_t)__popcll(need);
         const uint32_t take = avail < nneed ? (uint32_t)avail : nneed;
         const uint32_t rank = lane_rank(need);
-        if (phase == PH_NEED && rank < take) {
+        const bool got = phase == PH_NEED && rank < take;
+        if (got) {
           pid = win_next + rank;
           set = cur_set;
           stream = win_stream;
-          pf = philox_fixed((uint32_t)pid, (uint32_t)(pid >> 32), stream, k0, k1);
-          outidx = win_outbase + (pid - win_first);
-          w = (R)1;
-          muz = pts[set].mu0_z;
-          side = 0;
-          A = (R)1;
-          L_thr = pts[set].roul_thr;
-          ev = 0;
+          if (kPerPacket) outidx = win_outbase + (pid - win_first);
           phase = PH_ENTER;
         }
+        fresh |= __ballot(got);
         win_next += take;
         if (cur_set) outstanding1 += take; else outstanding0 += take;
         need = __ballot(phase == PH_NEED);
+      }
+      if (__builtin_amdgcn_inverse_ballot_w64(fresh)) {
+        pf = philox_fixed((uint32_t)pid, (uint32_t)(pid >> 32), stream, k0, k1);
+        w = (R)1;
+        muz = pts[set].mu0_z;
+        side = 0;
+        A = (R)1;
+        L_thr = pts[set].roul_thr;
+        ev = 0;
       }
     }
 
