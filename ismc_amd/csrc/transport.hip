@@ -93,7 +93,8 @@ constexpr uint64_t kMaxPacketsPerPoint = 1ull << 31;
 template <typename R> struct FlightSteps { static constexpr int value = 4; };
 template <> struct FlightSteps<float> { static constexpr int value = 6; };
 // waves per SIMD the FP32 tally kernel is compiled for at least (it takes
-// 64 VGPRs, no spills, so 8 fit; 7 and 8 measured the same); the FP64
+// 67 VGPRs, no spills, so 7 fit; a bound of 8 caps it at 64 VGPRs with 12
+// bytes per lane spilled); the FP64
 // kernel needs about twice the registers and the per-packet output
 // variant a few more, both are left to the compiler
 template <typename R, bool kPerPacket> struct MinWaves { static constexpr int value = 1; };
