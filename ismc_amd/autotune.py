@@ -92,15 +92,24 @@ class AutoTuner:
         max_chunk = int(counts.max()) if npts else 1
         chunks = [c for c in self.chunks if c <= max(max_chunk, 512)]
         table = []
+        # the C records are built once: every timed launch reads the same
+        # array, so a candidate costs its kernel time and little else
+        from .engine import config_array
+        arr = config_array(cfgs)
+        eng = self.engine
+
+        def launch(**shape):
+            eng.simulate_batch_array(arr, counts, seed=seed, streams=streams, fp64=fp64, **shape)
+            return eng.last_launch()["kernel_ms"]
+
         # first launch of a shape pays module load and buffer growth: discard one
-        self.engine.simulate_batch(cfgs, counts, seed=seed, streams=streams, fp64=fp64)
+        launch()
 
         def timed(shape):
             for t in table:
                 if all(t[k] == v for k, v in shape.items()):
                     return t
-            ms = min(self.engine.simulate_batch(cfgs, counts, seed=seed, streams=streams, fp64=fp64,
-                                                **shape)[0].kernel_ms for _ in range(self.repeats))
+            ms = min(launch(**shape) for _ in range(self.repeats))
             t = dict(shape, kernel_ms=ms, packets_per_s=total / (ms * 1e-3) if ms > 0 else float("inf"))
             table.append(t)
             return t

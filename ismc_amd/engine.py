@@ -58,6 +58,30 @@ def _result(t: _lib.Tally, kernel_ms=0.0) -> Result:
                   n_aborted=int(t.n_aborted), kernel_ms=kernel_ms)
 
 
+def _results(outs: np.ndarray, kernel_ms=0.0):
+    """_result for every row of a TALLY_DTYPE array, the arithmetic done on
+    the whole array at once."""
+    n = outs["n_packets"].astype(np.int64)
+    s = outs["sum"].astype(np.float64)
+    q = outs["sumsq"].astype(np.float64)
+    nf = np.maximum(n, 1).astype(np.float64)[:, None]
+    mean = np.where(n[:, None] > 0, s / nf, 0.0)
+    var = np.where(n[:, None] > 0, np.maximum(q / nf - mean ** 2, 0.0), 0.0)
+    se = np.sqrt(var / np.maximum(nf - 1.0, 1.0))
+    rows = zip(n.tolist(), mean.tolist(), se.tolist(), s.tolist(), q.tolist(),
+               outs["events"].tolist(), outs["n_aborted"].tolist())
+    return [Result(n_packets=ni, mean=dict(zip(TALLIES, m)), se=dict(zip(TALLIES, e)),
+                   sum=dict(zip(TALLIES, si)), sumsq=dict(zip(TALLIES, qi)),
+                   events=dict(zip(EVENTS, ev)), n_aborted=ab, kernel_ms=kernel_ms)
+            for ni, m, e, si, qi, ev, ab in rows]
+
+
+def config_array(cfgs: Sequence[SampleConfig]) -> np.ndarray:
+    """The configurations as one C-contiguous CONFIG_DTYPE array (the C
+    struct of include/ismc.h, one record per configuration)."""
+    return np.frombuffer(b"".join(bytes(c.to_c()) for c in cfgs), dtype=CONFIG_DTYPE).copy()
+
+
 class Engine:
     """Owns one libismc context on one GPU.  Not thread-safe: use one
     Engine per host thread."""
@@ -149,7 +173,7 @@ class Engine:
             blocks_per_cu, chunk_packets = kw["blocks_per_cu"], kw["chunk_packets"]
             idle_lanes = kw.get("idle_lanes", idle_lanes)
             bound_lanes = kw.get("bound_lanes", bound_lanes)
-        arr = (_lib.Config * npts)(*[c.to_c() for c in cfgs])
+        arr = config_array(cfgs)
         npk = np.ascontiguousarray(np.broadcast_to(np.asarray(n_packets, dtype=np.uint64), (npts,)))
         first = None
         if first_packet_ids is not None:
@@ -159,17 +183,17 @@ class Engine:
             st = np.ascontiguousarray(np.broadcast_to(np.asarray(streams, dtype=np.uint32), (npts,)))
         tune = _lib.Tune(int(blocks_per_cu), int(chunk_packets), int(bool(use_graph)), int(idle_lanes),
                          int(bound_lanes), 0)
-        outs = (_lib.Tally * npts)()
+        outs = np.zeros(npts, dtype=TALLY_DTYPE)
         P64 = ctypes.POINTER(ctypes.c_uint64)
         P32 = ctypes.POINTER(ctypes.c_uint32)
         rc = self._L.ismc_simulate_batch(
-            self._ctx, arr, npk.ctypes.data_as(P64),
+            self._ctx, arr.ctypes.data_as(ctypes.POINTER(_lib.Config)), npk.ctypes.data_as(P64),
             first.ctypes.data_as(P64) if first is not None else None,
             st.ctypes.data_as(P32) if st is not None else None,
-            npts, int(seed), self._flags(fp64, cuda_roulette), ctypes.byref(tune), outs)
+            npts, int(seed), self._flags(fp64, cuda_roulette), ctypes.byref(tune),
+            outs.ctypes.data_as(ctypes.POINTER(_lib.Tally)))
         _lib.check(self._ctx, rc, "ismc_simulate_batch")
-        ms = self.last_launch()["kernel_ms"]
-        return [_result(outs[i], ms) for i in range(npts)]
+        return _results(outs, self.last_launch()["kernel_ms"])
 
     def simulate_batch_array(self, configs: np.ndarray, n_packets, seed: int = 1234, streams=None,
                              fp64: bool = False, use_graph: bool = False, idle_lanes: int = 0,

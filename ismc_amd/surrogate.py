@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from .config import Layer, SampleConfig
-from .engine import Engine, default_engine
+from .engine import Engine, config_array, default_engine
 from .workloads import standard_sphere
 
 # input box (uniform in these coordinates): log10 mu_a, log10 mu_s, g, n, t
@@ -72,7 +72,9 @@ def generate_dataset(n_samples, packets, seed=0, engine: Optional[Engine] = None
     with return_se the labels' MC standard errors [n, 4] before the seconds
     (R_tot's from E[R_d^2] + E[R_s^2]: a packet leaves its weight in one
     exit tally).  A `stats` dict, when given, receives the launches' kernel
-    time ("kernel_ms") and summed event counts ("events")."""
+    time ("kernel_ms"), summed event counts ("events") and the seconds the
+    autotuner spent timing launch shapes on first use of this workload
+    class ("tuning_s", part of the returned seconds)."""
     eng = engine or default_engine()
     inp = sample_inputs(n_samples, seed)
     tal = np.zeros((n_samples, 4))
@@ -82,23 +84,29 @@ def generate_dataset(n_samples, packets, seed=0, engine: Optional[Engine] = None
         e = min(n_samples, s + points_per_launch)
         cfgs = [sample_config(inp["mu_a"][i], inp["mu_s"][i], inp["g"][i], inp["n"][i], inp["t"][i])
                 for i in range(s, e)]
-        res = eng.simulate_batch(cfgs, packets, seed=seed + 7, streams=np.arange(s, e, dtype=np.uint32),
-                                 chunk_packets=0 if autotune else min(int(packets), 1 << 20),
-                                 autotune=autotune)
+        kw = {"chunk_packets": min(int(packets), 1 << 20)}
+        if autotune:
+            t1 = time.perf_counter()
+            kw = eng.tuner.params(cfgs, packets, seed=seed + 7)
+            if stats is not None:
+                stats["tuning_s"] = stats.get("tuning_s", 0.0) + time.perf_counter() - t1
+        sums, sumsq, npk = eng.simulate_batch_array(config_array(cfgs), packets, seed=seed + 7,
+                                                    streams=np.arange(s, e, dtype=np.uint32), **kw)
         if stats is not None:
             stats["kernel_ms"] = stats.get("kernel_ms", 0.0) + eng.last_launch()["kernel_ms"]
             ev = stats.setdefault("events", {})
-            for r in res:
-                for name, v in r.events.items():
-                    ev[name] = ev.get(name, 0) + v
-        for k, r in enumerate(res):
-            m = r.mean
-            tal[s + k] = (m["R_d"], m["T_d"], m["T_u"], m["R_d"] + m["R_s"])
-            if return_se:
-                N = float(r.n_packets)
-                rt = m["R_d"] + m["R_s"]
-                v_tot = max((r.sumsq["R_d"] + r.sumsq["R_s"]) / N - rt * rt, 0.0)
-                se[s + k] = (r.se["R_d"], r.se["T_d"], r.se["T_u"], (v_tot / max(N - 1, 1)) ** 0.5)
+            for name, v in eng.last_events().items():
+                ev[name] = ev.get(name, 0) + v
+        # columns of sums in TALLIES order: A, R_d, R_s, T_d, T_u
+        N = np.maximum(npk.astype(np.float64), 1.0)[:, None]
+        m = sums / N
+        tal[s:e] = np.stack([m[:, 1], m[:, 3], m[:, 4], m[:, 1] + m[:, 2]], axis=1)
+        if return_se:
+            rt = m[:, 1] + m[:, 2]
+            v = np.maximum(sumsq / N - m * m, 0.0)
+            v_tot = np.maximum((sumsq[:, 1] + sumsq[:, 2]) / N[:, 0] - rt * rt, 0.0)
+            d = np.maximum(N - 1.0, 1.0)
+            se[s:e] = np.sqrt(np.stack([v[:, 1], v[:, 3], v[:, 4], v_tot], axis=1) / d)
         if progress:
             progress(e, n_samples, time.perf_counter() - t0)
     if return_se:
@@ -331,7 +339,8 @@ def holdout_report(sur, n_samples=16384, packets=100_000, seed=HOLDOUT_SEED, eng
     against the labels, the labels' median and RMS MC standard error, their
     ratio, and the network's own RMSE with the labels' noise removed,
     sqrt(max(RMSE^2 - mean SE^2, 0)).  Also the label generation's
-    packets/s."""
+    seconds and packets/s without the autotuner's one-time timing of this
+    workload class (reported beside them), and its kernel time."""
     eng = engine or default_engine()
     st = {}
     inp, tal, se, secs, npk = generate_dataset(n_samples, packets, seed=seed, engine=eng, return_se=True,
@@ -347,8 +356,10 @@ def holdout_report(sur, n_samples=16384, packets=100_000, seed=HOLDOUT_SEED, eng
                   "rmse_over_median_se": rmse / med if med > 0 else None,
                   "rmse_over_rms_se": rmse / msq ** 0.5 if msq > 0 else None,
                   "network_rmse_noise_removed": max(rmse * rmse - msq, 0.0) ** 0.5}
+    gen = secs - st.get("tuning_s", 0.0)
     return {"samples": n_samples, "packets_per_label": int(packets), "seed": seed,
-            "label_packets": int(npk), "label_seconds": secs, "label_packets_per_s": npk / secs,
+            "label_packets": int(npk), "label_seconds": gen, "label_packets_per_s": npk / gen,
+            "label_tuning_seconds": st.get("tuning_s", 0.0),
             "label_kernel_seconds": st["kernel_ms"] / 1e3,
             "label_kernel_packets_per_s": npk / (st["kernel_ms"] / 1e3), "label_events": st["events"],
             "outputs": out, "inputs": inp, "labels": tal, "label_se": se}

@@ -109,6 +109,21 @@ def test_surrogate_pipeline_small(engine):
     assert info["holdout_rmse"]["R_d"] < 0.05 and info["holdout_rmse"]["T_d"] < 0.05
 
 
+def test_labels_equal_simulate_batch_results(engine):
+    """generate_dataset (tuned launch shape, tallies read as arrays) gives
+    the same labels as simulate_batch's per-point Results of a default-shaped
+    launch on the same samples, seed and streams: tallies do not depend on
+    the launch shape."""
+    inp, tal, se, secs, npk = S.generate_dataset(600, 3000, seed=21, engine=engine, return_se=True)
+    cfgs = [S.sample_config(inp["mu_a"][i], inp["mu_s"][i], inp["g"][i], inp["n"][i], inp["t"][i])
+            for i in range(600)]
+    res = engine.simulate_batch(cfgs, 3000, seed=21 + 7, streams=np.arange(600, dtype=np.uint32))
+    for i, r in enumerate(res):
+        m = r.mean
+        assert tal[i].tolist() == [m["R_d"], m["T_d"], m["T_u"], m["R_d"] + m["R_s"]], i
+        np.testing.assert_allclose(se[i, :3], [r.se["R_d"], r.se["T_d"], r.se["T_u"]], rtol=1e-12)
+
+
 def test_train_rejects_indices_outside_the_dataset():
     """ismc_mlp_train checks every batch index against the resident dataset
     on the host (an index outside it would make the gather read beyond the
