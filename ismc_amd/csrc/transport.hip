@@ -115,8 +115,14 @@ enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2 };
 template <typename R> struct GlassGuard;
 template <> struct GlassGuard<float> { static constexpr float value = 9.5367431640625e-07f; };  // 2^-20
 template <> struct GlassGuard<double> { static constexpr double value = 9.094947017729282e-13; };  // 2^-40
+// Phases of a lane.  The ones that wait for a rare pass (need a packet,
+// enter, sphere decision, end) are exactly those with bit 1 clear, so one
+// AND and one compare find them.
 enum : int32_t { PH_NEED = 0, PH_ENTER = 1, PH_FLIGHT = 2, PH_DONE = 3, PH_SPHERE = 4, PH_END = 5,
                  PH_BOUND = 6 };
+static_assert((PH_NEED & 2) == 0 && (PH_ENTER & 2) == 0 && (PH_SPHERE & 2) == 0 && (PH_END & 2) == 0 &&
+                  (PH_FLIGHT & 2) && (PH_DONE & 2) && (PH_BOUND & 2),
+              "rare-pass phases are the ones with bit 1 clear");
 
 // Constants of the interface a packet meets when it leaves a layer in one
 // direction (towards the neighbouring layer, or the ambient medium at the
@@ -140,8 +146,8 @@ struct alignas(16) Iface {
 template <typename R>
 struct alignas(16) Layer {
   R zlo, zhi, n, step_k;     // step_k = ln2 / mu_t: s = -log2(u) * step_k
-  R absfrac, hg_a, hg_b, hg_c;  // mu_a/mu_t; 1+g^2; 1-g^2; 1-g (FP64) or 1+3g (FP32, see Draw)
-  R hg_d, hg_e, mu_t;        // 2g; 1/(2g); mu_t
+  R absfrac, hg_a, hg_b, hg_c;  // mu_a/mu_t; scattering constants (Draw<R>::constants)
+  R hg_d, hg_e, mu_t;
   int32_t kind;
   Iface<R> out[2];           // leaving towards lower z (0) / higher z (1)
 };
@@ -230,24 +236,47 @@ __device__ __forceinline__ R u01(uint32_t x) {
 // A draw as the flight step keeps it.  FP32 keeps the float m = 2 - u =
 // 1 + (x >> 9) 2^-23 in [1, 2) (one v_alignbit_b32) and forms u = 2 - m
 // only where u itself is read: the Henyey-Greenstein and azimuth formulas
-// take m directly, hc + hd u = (1 + 3g) - 2g m (the layer's hg_c holds
-// 1 + 3g), 2u - 1 = 3 - 2m, and cos 2 pi u = cos 2 pi m, which saves the
-// subtraction of two of each event's three uniforms.  FP64 keeps u and the
-// formulas written with it, operation for operation as the oracle has them.
+// take m directly, 1 - g + 2g u = (1 + 3g) - 2g m, 2u - 1 = 3 - 2m, and
+// cos 2 pi u = cos 2 pi m, which saves the subtraction of two of each
+// event's three uniforms.  FP64 keeps u and the formulas written with it,
+// operation for operation as the oracle has them.
+//
+// The FP32 scattering cosine is one formula for both kinds of layer,
+// cos(theta) = H - K r^2 + F m with r = 1 / ((1 + 3g) - 2g m):
+//   g != 0: H = (1 + g^2) / (2g), K = (1 - g^2)^2 / (2g), F = 0, which is
+//           the Henyey-Greenstein inverse CDF (1 + g^2 - q^2) / (2g) with
+//           q = (1 - g^2) r (CUDAISMLMC.h:284-285);
+//   g == 0: H = 3, K = 0, F = -2 (and r = 1), i.e. 3 - 2m = 2u - 1, the
+//           isotropic cosine (:284).
+// No per-event compare and select of the layer's kind; the layer record
+// holds (hg_a .. hg_e) = (H, K, 1 + 3g, 2g, F).  FP64 holds (1 + g^2,
+// 1 - g^2, 1 - g, 2g, 1/(2g)) and branches on the kind as the oracle does.
 template <typename R> struct Draw;
 template <> struct Draw<float> {
   static __device__ __forceinline__ float keep(uint32_t x) {
     return __builtin_bit_cast(float, __builtin_amdgcn_alignbit(0x7Fu, x, 9u));
   }
   static __device__ __forceinline__ float u(float m) { return 2.0f - m; }
-  static __device__ __forceinline__ float hg_cos(int kind, float ha, float hb, float hc3, float hd, float he,
-                                                 float m) {
-    const float q = hb * DM<float>::rcp(__builtin_fmaf(-hd, m, hc3));
-    const float c = (ha - q * q) * he;
-    return kind == KIND_ISO ? __builtin_fmaf(-2.0f, m, 3.0f) : c;
+  static __host__ __device__ void constants(double g, float* c) {
+    if (g == 0.0) {
+      c[0] = 3.0f; c[1] = 0.0f; c[2] = 1.0f; c[3] = 0.0f; c[4] = -2.0f;
+      return;
+    }
+    const double h = 1.0 / (2.0 * g), k = 1.0 - g * g;
+    c[0] = (float)((1.0 + g * g) * h);
+    c[1] = (float)(k * k * h);
+    c[2] = (float)(1.0 + 3.0 * g);
+    c[3] = (float)(2.0 * g);
+    c[4] = 0.0f;
   }
+  static __device__ __forceinline__ float hg_cos(int, float hh, float hk, float hc3, float hd, float hf,
+                                                 float m) {
+    const float r = DM<float>::rcp(__builtin_fmaf(-hd, m, hc3));
+    return __builtin_fmaf(hf, m, __builtin_fmaf(-hk, r * r, hh));
+  }
+  // the kept form of a uniform u given as a double (eval_kernel)
+  static __device__ __forceinline__ float from_u(double u) { return (float)(2.0 - u); }
   static __device__ __forceinline__ float cos2pi(float m) { return DM<float>::cos2pi(m); }
-  static constexpr bool kHc3 = true;   // hg_c holds 1 + 3g
 };
 
 // Unpolarised Fresnel reflectance for light going from index ni to nt at
@@ -262,13 +291,14 @@ template <> struct Draw<float> {
 template <typename R>
 __device__ __forceinline__ R fresnel(R eta, R qq, R mu_z, R& ct) {
   const R ci = DM<R>::abs(mu_z);
-  const R st2 = eta * eta * ((R)1 - ci * ci);
-  ct = DM<R>::sqrt(fmax((R)0, (R)1 - st2));
+  // ct^2 = 1 - sin^2(theta_t); not positive: total internal reflection
+  const R ct2 = (R)1 - eta * eta * ((R)1 - ci * ci);
+  ct = DM<R>::sqrt(fmax((R)0, ct2));
   const R a = eta * ci, c = eta * ct;
   const R rs = (a - ct) * DM<R>::rcp(a + ct);
   const R rp = (c - ci) * DM<R>::rcp(c + ci);
   R Rf = (R)0.5 * (rs * rs + rp * rp);
-  Rf = st2 >= (R)1 ? (R)1 : Rf;
+  Rf = ct2 <= (R)0 ? (R)1 : Rf;
   Rf = ci > (R)0.99999 ? qq : Rf;
   return Rf;
 }
@@ -278,11 +308,13 @@ __device__ __forceinline__ R fresnel(R eta, R qq, R mu_z, R& ct) {
 // air-side cosine; otherwise one air/sample interface.
 template <typename R>
 __device__ __forceinline__ R entrance_R(const Point<R>& P, int side, R mu_z, R& ct1) {
-  R ct2;
-  const R r1 = fresnel<R>(P.ent_eta1[side], P.ent_qq1[side], mu_z, ct1);
-  const R r2 = fresnel<R>(P.ent_eta2[side], P.ent_qq2[side], mu_z, ct2);
-  const R rg = r1 + ((R)1 - r1) * ((R)1 - r1) * r2 * DM<R>::rcp((R)1 - r1 * r2);
-  return P.ent_glass[side] ? rg : r1;
+  R r = fresnel<R>(P.ent_eta1[side], P.ent_qq1[side], mu_z, ct1);
+  if (P.ent_glass[side]) {  // the slide's far face only where there is a slide
+    R ct2;
+    const R r2 = fresnel<R>(P.ent_eta2[side], P.ent_qq2[side], mu_z, ct2);
+    r = r + ((R)1 - r) * ((R)1 - r) * r2 * DM<R>::rcp((R)1 - r * r2);
+  }
+  return r;
 }
 
 // Henyey-Greenstein cos(theta) (CUDAISMLMC.h:284-285).
@@ -296,12 +328,19 @@ __device__ __forceinline__ R hg_cos(int kind, R ha, R hb, R hc, R hd, R he, R u)
 template <> struct Draw<double> {
   static __device__ __forceinline__ double keep(uint32_t x) { return u01<double>(x); }
   static __device__ __forceinline__ double u(double v) { return v; }
+  static __host__ __device__ void constants(double g, double* c) {
+    c[0] = 1 + g * g;
+    c[1] = 1 - g * g;
+    c[2] = 1 - g;
+    c[3] = 2 * g;
+    c[4] = g == 0.0 ? 0.0 : 1 / (2 * g);
+  }
   static __device__ __forceinline__ double hg_cos(int kind, double ha, double hb, double hc, double hd,
                                                   double he, double u) {
     return ::ismc::hg_cos<double>(kind, ha, hb, hc, hd, he, u);
   }
+  static __device__ __forceinline__ double from_u(double u) { return u; }
   static __device__ __forceinline__ double cos2pi(double u) { return DM<double>::cos2pi(u); }
-  static constexpr bool kHc3 = false;  // hg_c holds 1 - g
 };
 
 // New mu_z after a scattering by theta with azimuth 2 pi u_phi, the
@@ -332,7 +371,21 @@ __device__ __forceinline__ R spin_z(R mu_z, R cos_t, R u_phi) {
 // per interface (Iface::cmin, Point::ent_cmin; 0 where no bound applies)
 template <typename R>
 __device__ __forceinline__ R glass_entry_cos(R ct, R cmin) {
-  return fmax(ct, cmin);
+  return ct > cmin ? ct : cmin;  // a compare and a select (fmax adds two canonicalizations)
+}
+
+// LDS records of a lane's point and layer.  Their byte offsets are 32-bit
+// and the indices small, so 24-bit multiplies (full rate) form them; plain
+// indexing compiles to v_mad_u64_u32 / v_mul_lo_u32 (half / quarter rate).
+template <typename R>
+__device__ __forceinline__ const Point<R>& point_at(const Point<R>* pts, int set) {
+  return *reinterpret_cast<const Point<R>*>(reinterpret_cast<const char*>(pts) +
+                                            __umul24((uint32_t)set, (uint32_t)sizeof(Point<R>)));
+}
+template <typename R>
+__device__ __forceinline__ const Layer<R>& layer_at(const Point<R>& P, int layer) {
+  return *reinterpret_cast<const Layer<R>*>(reinterpret_cast<const char*>(P.layer) +
+                                            __umul24((uint32_t)layer, (uint32_t)sizeof(Layer<R>)));
 }
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
@@ -442,7 +495,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     }
     const uint64_t fly = __ballot(phase == PH_FLIGHT);
     const uint64_t bnd = __ballot(phase == PH_BOUND);
-    const uint64_t idle = __ballot(phase != PH_FLIGHT && phase != PH_BOUND && phase != PH_DONE);
+    const uint64_t idle = __ballot((phase & 2) == 0);  // waiting for a rare pass
     if ((fly | bnd | idle) == 0) break;
     const bool rare = idle != 0 && ((fly | bnd) == 0 || (uint32_t)__popcll(idle) >= idle_thr);
     const bool bpass0 = bnd != 0 && (fly == 0 || (uint32_t)__popcll(bnd) >= bound_thr);
@@ -450,7 +503,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     if (rare) {
       // ---- sphere capture / return (CUDAISMLMC.h:377-410, 442-475, 518-550, 583-615)
       if (phase == PH_SPHERE) {
-        const Point<R>& P = pts[set];
+        const Point<R>& P = point_at<R>(pts, set);
         e_sph += 1;
         if (DM<R>::abs(muz) >= P.cos_spec[side]) {
           term = side ? 4 : 2;
@@ -574,10 +627,10 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       if (__builtin_amdgcn_inverse_ballot_w64(fresh)) {
         pf = philox_fixed((uint32_t)pid, (uint32_t)(pid >> 32), stream, k0, k1);
         w = (R)1;
-        muz = pts[set].mu0_z;
+        muz = point_at<R>(pts, set).mu0_z;
         side = 0;
         A = (R)1;
-        L_thr = pts[set].roul_thr;
+        L_thr = point_at<R>(pts, set).roul_thr;
         ev = 0;
       }
     }
@@ -586,10 +639,15 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     // at an interface, then one event per lane in flight (and, in the first
     // step after a rare pass, per entering lane); a lane that reaches a
     // sphere or its end stops for this iteration
+    // lanes in flight for the step's events: the rare pass leaves the lanes
+    // in flight as they were, so the first step starts from this
+    // iteration's ballot; a step's ballot is taken again only after an
+    // interface pass
+    uint64_t fmask = fly;
 #pragma unroll
     for (int step = 0; step < FlightSteps<R>::value; ++step) {
     if (step > 0 && near_limit) break;
-    const Point<R>& P = pts[set];
+    const Point<R>& P = point_at<R>(pts, set);
     bool bpass = bpass0;  // decided with this iteration's first ballots
     if (step > 0) {
       const uint64_t f = __ballot(phase == PH_FLIGHT);
@@ -598,10 +656,12 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       // 32-bit popcounts: scalar instructions only)
       bpass = (uint32_t)(__builtin_popcount((uint32_t)b) + __builtin_popcount((uint32_t)(b >> 32))) >=
               (f == 0 ? 1u : bound_thr);
+      fmask = f;
     }
     // ---------------- interface pass: the hits recorded since the last one
     // (CUDAISMLMC.h:180-257), with the draws of the hit's own event
-    if (bpass && phase == PH_BOUND) {
+    if (bpass) {
+    if (phase == PH_BOUND) {
       const bool up = muz > (R)0;  // the direction the packet meets the interface in
       const R zb = up ? L_zhi : L_zlo;
       const R dz = zb - z;
@@ -610,7 +670,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       z = zb;
       s = glass ? (R)0 : s - d;
       const int nxt = up ? layer + 1 : layer - 1;
-      const Iface<R> f = P.layer[layer].out[up ? 1 : 0];
+      const Iface<R> f = layer_at<R>(P, layer).out[up ? 1 : 0];
       const bool outside = f.mut < (R)0;  // nxt < 0 or nxt >= n_layers, from the host
       R ct;
       const R Rf = fresnel<R>(f.eta, f.qq, muz, ct);
@@ -630,7 +690,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         } else {
           // f.cmin is 0 unless the next layer is a denser slide, and f.mut
           // is 1 unless both layers scatter (in a slide s is 0 anyway)
-          const Layer<R>& NL = P.layer[nxt];
+          const Layer<R>& NL = layer_at<R>(P, nxt);
           muz = DM<R>::csign(glass_entry_cos<R>(ct, f.cmin), muz);
           s *= f.mut;
           layer = nxt;
@@ -647,10 +707,12 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         aborted = true;
       }
     }
+    fmask = __ballot(phase == PH_FLIGHT);
+    }
 
-    const bool run_flight = (phase == PH_FLIGHT);
+    const bool run_flight = __builtin_amdgcn_inverse_ballot_w64(fmask);
     const bool run_enter = step == 0 && rare && (phase == PH_ENTER);
-    if (__ballot(run_flight || run_enter) == 0) break;
+    if ((fmask | __ballot(run_enter)) == 0) break;
     uint32_t rw[4];
     philox_block(pf, ev, k0, k1, rw);
     // The event's first three uniforms go straight into the registers an
@@ -683,26 +745,29 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       // wave runs this code whenever one of its lanes interacts) and taken
       // where the lane interacted, which saves the copies that merging two
       // branches costs.
+      // A packet's weight reaches 0 only where roulette strikes (w - w
+      // mu_a/mu_t > 0 in a turbid layer down to the roulette threshold), so
+      // the roulette branch, which runs only when a lane's weight falls to
+      // the threshold, is where a packet ends; the new direction and step
+      // of a packet that ended are never read.
       R w_new = w - w * L_abs;
+      phase = hit ? PH_BOUND : PH_FLIGHT;
       if (!hit && w_new <= L_thr) {
         const bool survive = !P.cuda_roulette && (u3 <= P.roul_p);
         const R w_r = survive ? w_new * P.roul_m : (R)0;
         A += w_r - w_new;
         w_new = w_r;
+        if (!(w_r > (R)0)) phase = PH_END;
       }
       const R ct = Draw<R>::hg_cos(L_kind, L_ha, L_hb, L_hc, L_hd, L_he, u1);
       const R muz_new = spin_z<R>(muz, ct, u2);
       const R s_new = DM<R>::nlog2(u0) * L_step;
-      const bool live = w_new > (R)0;
       if (!hit) {
         z += s * muz;
         w = w_new;
-      }
-      if (!hit && live) {
         muz = muz_new;
         s = s_new;
       }
-      phase = hit ? PH_BOUND : (live ? PH_FLIGHT : PH_END);
       ev += 1;
     } else if (run_enter) {
       // (re-)entrance through the outer face (CUDAISMLMC.h:367-369, 502-503)
@@ -714,7 +779,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       } else {
         muz = DM<R>::csign(ct, muz);
         layer = P.first_layer[side];
-        const Layer<R>& NL = P.layer[layer];
+        const Layer<R>& NL = layer_at<R>(P, layer);
         L_zlo = NL.zlo; L_zhi = NL.zhi; L_step = NL.step_k; L_abs = NL.absfrac;
         L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
         L_kind = NL.kind;
@@ -757,10 +822,12 @@ __global__ void eval_kernel(int op, const Point<R>* pt, const double* in4, int n
     o0 = (double)fresnel<R>((R)(x[0] / x[1]), (R)(q * q), (R)x[2], ct);
     o1 = (double)ct;
   } else if (op == 1) {
-    const R g = (R)x[0];
-    const int kind = g == (R)0 ? KIND_ISO : KIND_HG;
-    o0 = (double)hg_cos<R>(kind, (R)1 + g * g, (R)1 - g * g, (R)1 - g, (R)2 * g,
-                           g == (R)0 ? (R)0 : (R)1 / ((R)2 * g), (R)x[1]);
+    // the scattering cosine as the transport kernel forms it, from the
+    // layer constants the host stores for g
+    R hc[5];
+    Draw<R>::constants(x[0], hc);
+    const int kind = x[0] == 0.0 ? KIND_ISO : KIND_HG;
+    o0 = (double)Draw<R>::hg_cos(kind, hc[0], hc[1], hc[2], hc[3], hc[4], Draw<R>::from_u(x[1]));
   } else if (op == 2) {
     o0 = (double)DM<R>::cos2pi((R)x[0]);
   } else if (op == 3) {
@@ -771,11 +838,11 @@ __global__ void eval_kernel(int op, const Point<R>* pt, const double* in4, int n
     o0 = (double)entrance_R<R>(*pt, mz > (R)0 ? 0 : 1, mz, ct);
     o1 = (double)ct;
   } else if (op == 5) {
-    const R g = (R)x[1];
-    const int kind = g == (R)0 ? KIND_ISO : KIND_HG;
-    const R c = hg_cos<R>(kind, (R)1 + g * g, (R)1 - g * g, (R)1 - g, (R)2 * g,
-                          g == (R)0 ? (R)0 : (R)1 / ((R)2 * g), (R)x[2]);
-    o0 = (double)spin_z<R>((R)x[0], c, (R)x[3]);
+    R hc[5];
+    Draw<R>::constants(x[1], hc);
+    const int kind = x[1] == 0.0 ? KIND_ISO : KIND_HG;
+    const R c = Draw<R>::hg_cos(kind, hc[0], hc[1], hc[2], hc[3], hc[4], Draw<R>::from_u(x[2]));
+    o0 = (double)spin_z<R>((R)x[0], c, Draw<R>::from_u(x[3]));
     o1 = (double)c;
   }
   out2[2 * i] = o0;
@@ -978,11 +1045,13 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
       Ly.kind = g == 0.0 ? KIND_ISO : KIND_HG;
       Ly.step_k = (R)(std::log(2.0) / mu_t);
       Ly.absfrac = (R)(cfg->mu_a[i] / mu_t);
-      Ly.hg_a = (R)(1 + g * g);
-      Ly.hg_b = (R)(1 - g * g);
-      Ly.hg_c = Draw<R>::kHc3 ? (R)(1 + 3 * g) : (R)(1 - g);
-      Ly.hg_d = (R)(2 * g);
-      Ly.hg_e = g == 0.0 ? (R)0 : (R)(1 / (2 * g));
+      R hc[5];
+      Draw<R>::constants(g, hc);
+      Ly.hg_a = hc[0];
+      Ly.hg_b = hc[1];
+      Ly.hg_c = hc[2];
+      Ly.hg_d = hc[3];
+      Ly.hg_e = hc[4];
       Ly.mu_t = (R)mu_t;
     }
   }
