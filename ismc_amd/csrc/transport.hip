@@ -134,9 +134,10 @@ struct alignas(16) Iface {
   R qq;    // ((n_there - n_here) / (n_there + n_here))^2, normal incidence
   R cmin;  // entering a glass slide from a less dense layer: the least |mu_z|
            // inside it (glass_entry_cos); 0 otherwise
-  R mut;   // mu_t here / mu_t there: rescales the remaining step s when both
-           // layers scatter; -1 where the packet leaves the stack (the
-           // interface pass reads "outside" from its sign, no layer count)
+  R mut;   // rescales the remaining step s on transmission: mu_t here /
+           // mu_t there when both layers scatter, 0 out of a slide, 1 into
+           // one; -1 where the packet leaves the stack (the interface pass
+           // reads "outside" from its sign, no layer count)
 };
 
 // Constants of one layer, derived on the host in FP64.  n and mu_t are not
@@ -145,7 +146,8 @@ struct alignas(16) Iface {
 // they stay as the layer's record.
 template <typename R>
 struct alignas(16) Layer {
-  R zlo, zhi, n, step_k;     // step_k = ln2 / mu_t: s = -log2(u) * step_k
+  R half, zc, n, step_k;     // half thickness; centre depth (record only); step_k = ln2 / mu_t:
+                             // s = -log2(u) * step_k
   R absfrac, hg_a, hg_b, hg_c;  // mu_a/mu_t; scattering constants (Draw<R>::constants)
   R hg_d, hg_e, mu_t;
   int32_t kind;
@@ -294,10 +296,13 @@ __device__ __forceinline__ R fresnel(R eta, R qq, R mu_z, R& ct) {
   // ct^2 = 1 - sin^2(theta_t); not positive: total internal reflection
   const R ct2 = (R)1 - eta * eta * ((R)1 - ci * ci);
   ct = DM<R>::sqrt(fmax((R)0, ct2));
+  // rs = (a - ct) / (a + ct), rp = (c - ci) / (c + ci) and R = (rs^2 +
+  // rp^2) / 2 over one reciprocal: a transcendental takes four full-rate
+  // slots, and B D >= min(ci, ct)^2 stays far above FP32's underflow
   const R a = eta * ci, c = eta * ct;
-  const R rs = (a - ct) * DM<R>::rcp(a + ct);
-  const R rp = (c - ci) * DM<R>::rcp(c + ci);
-  R Rf = (R)0.5 * (rs * rs + rp * rp);
+  const R A = a - ct, B = a + ct, C = c - ci, D = c + ci;
+  const R AD = A * D, CB = C * B, BD = B * D;
+  R Rf = (AD * AD + CB * CB) * DM<R>::rcp((R)2 * (BD * BD));
   Rf = ct2 <= (R)0 ? (R)1 : Rf;
   Rf = ci > (R)0.99999 ? qq : Rf;
   return Rf;
@@ -382,10 +387,17 @@ __device__ __forceinline__ const Point<R>& point_at(const Point<R>* pts, int set
   return *reinterpret_cast<const Point<R>*>(reinterpret_cast<const char*>(pts) +
                                             __umul24((uint32_t)set, (uint32_t)sizeof(Point<R>)));
 }
+// A lane keeps its current layer as the byte offset of that layer's record
+// from the wave's point slots (slot set, layer index): an interface reads
+// its record and steps to the neighbouring layer by adding offsets.
 template <typename R>
-__device__ __forceinline__ const Layer<R>& layer_at(const Point<R>& P, int layer) {
-  return *reinterpret_cast<const Layer<R>*>(reinterpret_cast<const char*>(P.layer) +
-                                            __umul24((uint32_t)layer, (uint32_t)sizeof(Layer<R>)));
+__device__ __forceinline__ uint32_t layer_offset(int set, int layer) {
+  return __umul24((uint32_t)set, (uint32_t)sizeof(Point<R>)) + (uint32_t)offsetof(Point<R>, layer) +
+         __umul24((uint32_t)layer, (uint32_t)sizeof(Layer<R>));
+}
+template <typename R>
+__device__ __forceinline__ const Layer<R>& layer_rec(const Point<R>* pts, uint32_t lofs) {
+  return *reinterpret_cast<const Layer<R>*>(reinterpret_cast<const char*>(pts) + lofs);
 }
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
@@ -446,12 +458,17 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
   bool exhausted = false;
 
   // lane state
-  int phase = PH_NEED, side = 0, layer = 0, set = 0, term = 0;
+  int phase = PH_NEED, side = 0, set = 0, term = 0;
+  uint32_t lofs = 0;  // the current layer (layer_offset)
   bool aborted = false;
   // A is the packet's starting weight plus the weight Russian roulette has
   // added or removed: the weight absorbed so far is A - w (every other
   // change of w is an absorption), so an interaction updates w alone
-  R w = 0, z = 0, muz = 1, s = 0, A = 0;
+  // zr is the packet's depth relative to the centre of its layer, whose
+  // faces are then at -half and +half exactly: a flight reaches the face
+  // ahead when |zr + s mu_z| > half (two instructions), and a packet on a
+  // face (zr = +-half) with a zero step stays inside
+  R w = 0, zr = 0, muz = 1, s = 0, A = 0;
   R su1 = 0, su2 = 0;  // return draw and new |mu_z| of a pending sphere decision
   R bu0 = 0;           // Fresnel draw of a pending interface hit
   uint64_t pid = 0, outidx = 0;
@@ -461,7 +478,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
   // (every event is an interaction, an interface hit or an entrance)
   uint32_t e_bnd = 0, e_inc = 0, e_sph = 0, e_rei = 0;
   // current layer cache
-  R L_zlo = 0, L_zhi = 0, L_step = 0, L_abs = 0, L_ha = 0, L_hb = 0, L_hc = 0, L_hd = 0,
+  R L_half = 0, L_step = 0, L_abs = 0, L_ha = 0, L_hb = 0, L_hc = 0, L_hd = 0,
     L_he = 0;
   int L_kind = KIND_GLASS;
   R L_thr = 0;  // the point's roulette threshold
@@ -663,15 +680,15 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     if (bpass) {
     if (phase == PH_BOUND) {
       const bool up = muz > (R)0;  // the direction the packet meets the interface in
-      const R zb = up ? L_zhi : L_zlo;
-      const R dz = zb - z;
-      const bool glass = (L_kind == KIND_GLASS);
+      const R zb = DM<R>::csign(L_half, muz);  // the face met
+      const R dz = zb - zr;
       const R d = dz * DM<R>::rcp(muz);
-      z = zb;
-      s = glass ? (R)0 : s - d;
-      const int nxt = up ? layer + 1 : layer - 1;
-      const Iface<R> f = layer_at<R>(P, layer).out[up ? 1 : 0];
-      const bool outside = f.mut < (R)0;  // nxt < 0 or nxt >= n_layers, from the host
+      zr = zb;
+      // the step left (in a slide s is never read, and its transmission
+      // into the next layer multiplies it by f.mut = 0)
+      s = s - d;
+      const Iface<R> f = layer_rec<R>(pts, lofs).out[up ? 1 : 0];
+      const bool outside = f.mut < (R)0;  // no layer beyond (from the host)
       R ct;
       const R Rf = fresnel<R>(f.eta, f.qq, muz, ct);
       e_bnd += 1;
@@ -688,13 +705,16 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
           side = up ? 1 : 0;
           phase = PH_SPHERE;  // su1, su2 hold the hit's return draw and new |mu_z|
         } else {
-          // f.cmin is 0 unless the next layer is a denser slide, and f.mut
-          // is 1 unless both layers scatter (in a slide s is 0 anyway)
-          const Layer<R>& NL = layer_at<R>(P, nxt);
+          // f.cmin is 0 unless the next layer is a denser slide; f.mut is
+          // mu_t here / mu_t there where both layers scatter, 0 out of a
+          // slide (the zero step into a scattering layer, CUDAISMLMC.h:
+          // 230-235) and 1 into one
+          lofs = up ? lofs + (uint32_t)sizeof(Layer<R>) : lofs - (uint32_t)sizeof(Layer<R>);
+          const Layer<R>& NL = layer_rec<R>(pts, lofs);
           muz = DM<R>::csign(glass_entry_cos<R>(ct, f.cmin), muz);
           s *= f.mut;
-          layer = nxt;
-          L_zlo = NL.zlo; L_zhi = NL.zhi; L_step = NL.step_k; L_abs = NL.absfrac;
+          zr = up ? -NL.half : NL.half;  // the near face of the next layer
+          L_half = NL.half; L_step = NL.step_k; L_abs = NL.absfrac;
           L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
           L_kind = NL.kind;
         }
@@ -729,16 +749,16 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     const R u0 = bu0, u1 = su1, u2 = su2, u3 = u01<R>(rw[3]);
 
     if (run_flight) {
-      const R zb = muz > (R)0 ? L_zhi : L_zlo;
-      const R dz = zb - z;
       const bool glass = (L_kind == KIND_GLASS);
       // The reference hits when s >= d = dz / mu_z, with d infinite for
-      // mu_z = 0 (MLMC.py:100-114).  Here s |mu_z| > |dz|: the same test
-      // except where the two are exactly equal (an interaction that ends on
-      // the interface, whose next event is then the hit), and mu_z = 0 never
-      // hits without a compare of its own (1.2 % faster).  Bitwise, not
-      // short-circuit: no branches around the comparisons.
-      const bool hit = glass | (s * DM<R>::abs(muz) > DM<R>::abs(dz));
+      // mu_z = 0 (MLMC.py:100-114).  Here the hop's end zr + s mu_z lies
+      // beyond the face ahead: s |mu_z| > |dz|, the same test except where
+      // the two are exactly equal (an interaction that ends on the
+      // interface, whose next event is then the hit), and mu_z = 0 never
+      // hits.  Bitwise, not short-circuit: no branches around the
+      // comparisons.
+      const R zn = s * muz + zr;
+      const bool hit = glass | (DM<R>::abs(zn) > L_half);
       // A hit reaches the interface (its draws wait in bu0, su1, su2);
       // otherwise hop, drop, roulette, spin (CUDAISMLMC.h:259-309).  The
       // interaction's new state is computed for every lane in flight (the
@@ -763,7 +783,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       const R muz_new = spin_z<R>(muz, ct, u2);
       const R s_new = DM<R>::nlog2(u0) * L_step;
       if (!hit) {
-        z += s * muz;
+        zr = zn;
         w = w_new;
         muz = muz_new;
         s = s_new;
@@ -778,13 +798,13 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
         phase = PH_SPHERE;  // su1, su2 hold this event's return draw and new |mu_z|
       } else {
         muz = DM<R>::csign(ct, muz);
-        layer = P.first_layer[side];
-        const Layer<R>& NL = layer_at<R>(P, layer);
-        L_zlo = NL.zlo; L_zhi = NL.zhi; L_step = NL.step_k; L_abs = NL.absfrac;
+        lofs = layer_offset<R>(set, P.first_layer[side]);
+        const Layer<R>& NL = layer_rec<R>(pts, lofs);
+        L_half = NL.half; L_step = NL.step_k; L_abs = NL.absfrac;
         L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
         L_kind = NL.kind;
         muz = DM<R>::csign(glass_entry_cos<R>(ct, P.ent_cmin[side]), muz);
-        z = side ? P.t_tot : (R)0;
+        zr = side ? NL.half : -NL.half;  // the stack's face: first layer's bottom / last's top
         s = L_kind == KIND_GLASS ? (R)0 : DM<R>::nlog2(Draw<R>::u(u1)) * L_step;
         phase = PH_FLIGHT;
       }
@@ -1021,7 +1041,10 @@ static Iface<R> make_iface(double na, double nb, bool a_turbid, double ta, bool 
     cmin = std::sqrt(std::max(0.0, 1.0 - (1.0 - (double)GlassGuard<R>::value) * r * r));
   }
   f.cmin = (R)cmin;
-  f.mut = (a_turbid && b_turbid) ? (R)(ta / tb) : (R)1;
+  // rescales the remaining step on transmission: mu_t(a) / mu_t(b) between
+  // two scattering layers, 0 out of a slide (its step is never read; the
+  // first step in the layer behind is the zero step), 1 into a slide
+  f.mut = a_turbid ? (b_turbid ? (R)(ta / tb) : (R)1) : (R)0;
   return f;
 }
 
@@ -1035,9 +1058,9 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
   for (int i = 0; i < L; ++i) {
     Layer<R>& Ly = P->layer[i];
     const double mu_t = cfg->mu_t[i], g = cfg->g[i];
-    Ly.zlo = (R)z;
+    Ly.zc = (R)(z + 0.5 * cfg->t[i]);
+    Ly.half = (R)(0.5 * cfg->t[i]);
     z += cfg->t[i];
-    Ly.zhi = (R)z;
     Ly.n = (R)cfg->n[i];
     if (cfg->mu_s[i] == 0.0) {
       Ly.kind = KIND_GLASS;
