@@ -204,8 +204,15 @@ template <> struct DM<float> {
   static __device__ __forceinline__ float cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
   static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
   static __device__ __forceinline__ float csign(float a, float b) { return __builtin_copysignf(a, b); }
+  // round(v 2^32) for 0 <= v < 2^32 as the integer part (a truncating
+  // convert) and round(fract(v) 2^32), which is below 2^32 - 255 for an FP32
+  // fraction: the same value as rint(v 2^32) converted to 64 bits (adding
+  // the even number floor(v) 2^32 changes neither the rounding nor its tie
+  // rule), in five instructions instead of seven
   static __device__ __forceinline__ unsigned long long fx(float v) {
-    return (unsigned long long)__builtin_rintf(v * 4294967296.0f);
+    const uint32_t hi = (uint32_t)v;
+    const uint32_t lo = (uint32_t)__builtin_rintf(__builtin_amdgcn_fractf(v) * 4294967296.0f);
+    return ((unsigned long long)hi << 32) | lo;
   }
 };
 template <> struct DM<double> {
