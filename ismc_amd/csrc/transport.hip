@@ -100,6 +100,18 @@ template <> struct FlightSteps<float> { static constexpr int value = 6; };
 template <typename R, bool kPerPacket> struct MinWaves { static constexpr int value = 1; };
 template <> struct MinWaves<float, false> { static constexpr int value = 7; };
 constexpr int kSlotWords = 16;  // per point: sum[5], sumsq[5], events[5], aborted
+// A wave's LDS tally accumulator is kept in kAccCopies interleaved copies
+// (word k of copy c at k * kAccCopies + c, so a word's copies sit in
+// consecutive 8-byte slots): a lane that ends a packet adds to copy
+// lane % kAccCopies.  The lanes ending in one rare pass all add to the same
+// words, and same-address LDS atomics of one instruction serialise; four
+// copies spread them over four addresses.  Measured on one layer with
+// mu_s 10 (short histories, many packet ends per step): 17.6 % less kernel
+// time with four copies, the same with eight; the longer workloads within
+// 0.2 % with four and up to 1.4 % slower with eight (more registers).  A
+// slot's flush sums the copies.
+constexpr int kAccCopies = 4;
+constexpr int kAccWords = kSlotWords * kAccCopies;
 
 enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2 };
 
@@ -432,23 +444,32 @@ __device__ __forceinline__ void flush_slot(unsigned long long* acc, unsigned lon
   uint32_t l = (uint32_t)lane;
   asm volatile("" : "+v"(l));
   if (l < (uint32_t)kSlotWords) {
-    const unsigned long long v = acc[l];
+    // one copy at a time (once per chunk; an unrolled sum would hold every
+    // copy in registers at the kernel's busiest point)
+    unsigned long long v = 0ull;
+#pragma unroll 1
+    for (int c = 0; c < kAccCopies; ++c) v += acc[l * kAccCopies + c];
     if (v) atomicAdd(&row[l], v);
-    acc[l] = 0ull;
   }
+  // every lane's reads are issued before any lane clears a copy (a wave's
+  // LDS operations execute in order)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t i = l; i < (uint32_t)kAccWords; i += 64) acc[i] = 0ull;
 }
 
 template <typename R, bool kPerPacket>
 __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void transport_kernel(Args a) {
   __shared__ Point<R> s_pt[kWavesPerBlock][2];
-  __shared__ unsigned long long s_acc[kWavesPerBlock][2][kSlotWords];
+  __shared__ unsigned long long s_acc[kWavesPerBlock][2][kAccWords];
 
   const int wid = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   Point<R>* pts = s_pt[wid];
   unsigned long long* acc0 = s_acc[wid][0];
   unsigned long long* acc1 = s_acc[wid][1];
-  if (lane < kSlotWords) { acc0[lane] = 0ull; acc1[lane] = 0ull; }
+  for (int i = lane; i < kAccWords; i += 64) { acc0[i] = 0ull; acc1[i] = 0ull; }
   __builtin_amdgcn_wave_barrier();
 
   const Point<R>* gpts = reinterpret_cast<const Point<R>*>(a.points);
@@ -545,24 +566,31 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       // ---- end of packet: tallies into the chunk slot's LDS accumulator
       const bool ending = (phase == PH_END);
       if (ending) {
-        unsigned long long* acc = set ? acc1 : acc0;
+        // this lane's copy of the slot's accumulator: word k at acc[k * kAccCopies]
+        // (the copy index passes through an empty asm operand so that the
+        // compiler forms the address here instead of holding two per-lane
+        // addresses in registers through the whole kernel)
+        uint32_t cpy = (uint32_t)lane & (uint32_t)(kAccCopies - 1);
+        asm volatile("" : "+v"(cpy));
+        unsigned long long* acc = (set ? acc1 : acc0) + cpy;
         A = aborted ? (R)0 : A - w;  // the absorbed weight
+        constexpr int C = kAccCopies;
         if (aborted) {
-          atomicAdd(&acc[15], 1ull);
+          atomicAdd(&acc[15 * C], 1ull);
           term = 0;
         } else {
-          atomicAdd(&acc[0], DM<R>::fx(A));
-          atomicAdd(&acc[5], DM<R>::fx(A * A));
+          atomicAdd(&acc[0 * C], DM<R>::fx(A));
+          atomicAdd(&acc[5 * C], DM<R>::fx(A * A));
           if (term) {
-            atomicAdd(&acc[term], DM<R>::fx(w));
-            atomicAdd(&acc[5 + term], DM<R>::fx(w * w));
+            atomicAdd(&acc[term * C], DM<R>::fx(w));
+            atomicAdd(&acc[(5 + term) * C], DM<R>::fx(w * w));
           }
         }
-        atomicAdd(&acc[10], (unsigned long long)(ev - e_bnd - e_inc));
-        atomicAdd(&acc[11], (unsigned long long)e_bnd);
-        atomicAdd(&acc[12], (unsigned long long)e_inc);
-        atomicAdd(&acc[13], (unsigned long long)e_sph);
-        atomicAdd(&acc[14], (unsigned long long)e_rei);
+        atomicAdd(&acc[10 * C], (unsigned long long)(ev - e_bnd - e_inc));
+        atomicAdd(&acc[11 * C], (unsigned long long)e_bnd);
+        atomicAdd(&acc[12 * C], (unsigned long long)e_inc);
+        atomicAdd(&acc[13 * C], (unsigned long long)e_sph);
+        atomicAdd(&acc[14 * C], (unsigned long long)e_rei);
         if (kPerPacket) {
           double* o = a.per_packet + outidx * 6;
           o[0] = (double)A;
