@@ -232,9 +232,9 @@ def inverse_bench(eng, count):
     out = {}
     for key, methods, kw in (
             ("noise_1pct", ("newton", "praxis"),
-             dict(count=count, noise=0.01, plain=((1, 512, 48), (5, 128, 8)))),
+             dict(count=count, noise=0.01, plain=((1, 512, 96), (5, 128, 8)))),
             ("noise_0.1pct", ("newton",),
-             dict(count=count, noise=0.001, plain=((1, 128, 16),), meas_packets=10_000_000,
+             dict(count=count, noise=0.001, plain=((1, 128, 32),), meas_packets=10_000_000,
                   refine_packets=1_000_000, check_packets=10_000_000, plain_packets=10_000_000,
                   plain_tol=1e-3))):
         r = inverse.run(sur, engine=eng, methods=methods, with_ablation=False, log=log, **kw)

@@ -640,6 +640,11 @@ def _plain_stats(probs, meas, cost_truth, runs, count, timed, packets, tol, chec
             "timed_as": "sequential: one PRAXIS per run, one %d-packet launch per cost evaluation, from "
                         "the generic guess%s" % (packets, "; mean of %d accepted runs" % runs if runs > 1 else ""),
             "solves_per_s": timed / sum(secs) if timed else None, "seconds_timed": sum(secs),
+            # a solve's cost is heavy-tailed (a path through long-history
+            # parameters can take seconds), so the rate is total solves over
+            # total time and the spread is reported beside it
+            "seconds_per_solve": ({"median": float(np.median(secs)), "p90": float(np.percentile(secs, 90)),
+                                   "max": float(np.max(secs))} if timed else None),
             "mean_evals_timed": float(np.mean(sevals)) if timed else None,
             "mean_evals": float(np.mean(nev)), "retries": int(retries.sum()),
             "sequential_equals_lockstep": same, "lockstep_s": t_batch, "packets_per_eval": packets,
