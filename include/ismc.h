@@ -42,6 +42,13 @@ extern "C" {
 /* flags */
 #define ISMC_FLAG_FP64 0x1u           /* FP64 transport (deviation reference)  */
 #define ISMC_FLAG_CUDA_ROULETTE 0x4u  /* CUDAISMLMC.h:275: int 1/m == 0, no survivor */
+/* Simulate a slide's bounces face by face, as MLMC.py:114-163 does.  By
+ * default a slide (a glass layer between scattering layers or the ambient
+ * medium) is taken in closed form: a packet crossing into it leaves back
+ * or through in one boundary event, with the same probabilities and
+ * directions as the bounce-by-bounce simulation; only the boundary-hit
+ * count differs. */
+#define ISMC_FLAG_SLIDE_BOUNCES 0x8u
 
 /* One integrating sphere (or, when !present, the angle threshold that
  * separates specular/unscattered from diffuse light; CUDAISMLMC.h:413-433). */

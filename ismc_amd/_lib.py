@@ -11,6 +11,7 @@ import threading
 MAX_LAYERS = 16
 FLAG_FP64 = 0x1
 FLAG_CUDA_ROULETTE = 0x4
+FLAG_SLIDE_BOUNCES = 0x8
 
 OK, EINVAL, EDEVICE, ENOMEM, ERANGE = 0, -1, -2, -3, -4
 _STATUS = {EINVAL: "invalid argument", EDEVICE: "HIP error", ENOMEM: "out of memory",

@@ -113,7 +113,11 @@ constexpr int kSlotWords = 16;  // per point: sum[5], sumsq[5], events[5], abort
 constexpr int kAccCopies = 4;
 constexpr int kAccWords = kSlotWords * kAccCopies;
 
-enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2 };
+// Layer kinds.  A slide is a glass layer whose neighbours are scattering
+// layers or the ambient medium: the kernel takes a packet's bounces inside
+// it in closed form (the interface pass and the entrance below); any other
+// glass layer is flown face by face.  Glass tests read kind >= KIND_GLASS.
+enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2, KIND_SLIDE = 3 };
 
 // A packet that refracts into a glass slide from a less dense medium a at
 // grazing incidence lands on the critical angle of the glass->a face.  In
@@ -147,9 +151,11 @@ struct alignas(16) Iface {
   R cmin;  // entering a glass slide from a less dense layer: the least |mu_z|
            // inside it (glass_entry_cos); 0 otherwise
   R mut;   // rescales the remaining step s on transmission: mu_t here /
-           // mu_t there when both layers scatter, 0 out of a slide, 1 into
-           // one; -1 where the packet leaves the stack (the interface pass
-           // reads "outside" from its sign, no layer count)
+           // mu_t there when both layers scatter, 0 out of a glass layer, 1
+           // into one; -1 where the packet leaves the stack (the interface
+           // pass reads "outside" from its sign, no layer count); +inf into
+           // a slide, whose bounces the pass takes in closed form (there
+           // the step is set, not rescaled)
 };
 
 // Constants of one layer, derived on the host in FP64.  n and mu_t are not
@@ -459,7 +465,11 @@ __device__ __forceinline__ void flush_slot(unsigned long long* acc, unsigned lon
   for (uint32_t i = l; i < (uint32_t)kAccWords; i += 64) acc[i] = 0ull;
 }
 
-template <typename R, bool kPerPacket>
+// kSlides: the launch has a slide in some point.  The closed form of a
+// slide's bounces is compiled into that instantiation only, so a launch
+// without slides runs code without it (its extra branches cost 1-2 % on
+// stacks without slides).
+template <typename R, bool kPerPacket, bool kSlides>
 __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void transport_kernel(Args a) {
   __shared__ Point<R> s_pt[kWavesPerBlock][2];
   __shared__ unsigned long long s_acc[kWavesPerBlock][2][kAccWords];
@@ -732,26 +742,61 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       // when u < R (MLMC.py:122-144); this u is in (0, 1], and with u <= R
       // the reflection probability is R on the draws' grid and total
       // internal reflection (R = 1) reflects always
+      constexpr uint32_t kLayerBytes = (uint32_t)sizeof(Layer<R>);
       if (bu0 <= Rf) {
         muz = -muz;
       } else {
+        const R muz0 = muz;  // the incident direction
         muz = DM<R>::csign(ct, muz);
         if (outside) {
           side = up ? 1 : 0;
           phase = PH_SPHERE;  // su1, su2 hold the hit's return draw and new |mu_z|
         } else {
-          // f.cmin is 0 unless the next layer is a denser slide; f.mut is
-          // mu_t here / mu_t there where both layers scatter, 0 out of a
-          // slide (the zero step into a scattering layer, CUDAISMLMC.h:
-          // 230-235) and 1 into one
-          lofs = up ? lofs + (uint32_t)sizeof(Layer<R>) : lofs - (uint32_t)sizeof(Layer<R>);
-          const Layer<R>& NL = layer_rec<R>(pts, lofs);
-          muz = DM<R>::csign(glass_entry_cos<R>(ct, f.cmin), muz);
-          s *= f.mut;
-          zr = up ? -NL.half : NL.half;  // the near face of the next layer
-          L_half = NL.half; L_step = NL.step_k; L_abs = NL.absfrac;
-          L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
-          L_kind = NL.kind;
+          // into the next layer; f.cmin is 0 unless it is a denser glass
+          // layer, f.mut is mu_t here / mu_t there where both layers
+          // scatter, 0 out of a glass layer (the zero step into a
+          // scattering layer, CUDAISMLMC.h:230-235), 1 into one
+          uint32_t nofs = up ? lofs + kLayerBytes : lofs - kLayerBytes;
+          const R cg = glass_entry_cos<R>(ct, f.cmin);  // the cosine in the next layer
+          bool enter = true;
+          if (kSlides && f.mut == (R)INFINITY) {  // the next layer is a slide
+            // A slide (lossless, plane-parallel) entered through this face
+            // (reflectance Rf, from either side by reciprocity) sends the
+            // packet on between this face and its far face (r2, at the
+            // slide's cosine) until it leaves: back through this face with
+            // probability r2 (1 - Rf) / (1 - Rf r2), through the far face
+            // otherwise, the geometric series of the bounces MLMC.py:114-163
+            // simulates one face at a time.  The hit's draw, rescaled to
+            // the event u > Rf, decides.  Either way the packet enters a
+            // scattering layer from glass: zero step (CUDAISMLMC.h:230-235).
+            const Iface<R> g = layer_rec<R>(pts, nofs).out[up ? 1 : 0];
+            R ct3;
+            const R r2 = fresnel<R>(g.eta, g.qq, cg, ct3);
+            s = (R)0;
+            if ((bu0 - Rf) * ((R)1 - Rf * r2) <= r2 * ((R)1 - Rf) * ((R)1 - Rf)) {
+              muz = -muz0;  // back into this layer, on the face
+              enter = false;
+            } else if (g.mut < (R)0) {
+              muz = DM<R>::csign(ct3, muz0);  // out of the stack through the slide
+              side = up ? 1 : 0;
+              phase = PH_SPHERE;
+              enter = false;
+            } else {
+              muz = DM<R>::csign(ct3, muz0);  // into the layer beyond the slide
+              nofs = up ? nofs + kLayerBytes : nofs - kLayerBytes;
+            }
+          } else {
+            muz = DM<R>::csign(cg, muz);
+            s *= f.mut;
+          }
+          if (enter) {
+            lofs = nofs;
+            const Layer<R>& NL = layer_rec<R>(pts, lofs);
+            zr = up ? -NL.half : NL.half;  // the near face of the next layer
+            L_half = NL.half; L_step = NL.step_k; L_abs = NL.absfrac;
+            L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
+            L_kind = NL.kind;
+          }
         }
       }
       // a packet still running after kMaxEventsPerPacket events is dropped
@@ -784,7 +829,7 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
     const R u0 = bu0, u1 = su1, u2 = su2, u3 = u01<R>(rw[3]);
 
     if (run_flight) {
-      const bool glass = (L_kind == KIND_GLASS);
+      const bool glass = (L_kind >= KIND_GLASS);
       // The reference hits when s >= d = dz / mu_z, with d infinite for
       // mu_z = 0 (MLMC.py:100-114).  Here the hop's end zr + s mu_z lies
       // beyond the face ahead: s |mu_z| > |dz|, the same test except where
@@ -832,16 +877,46 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       if (u0 < Re) {
         phase = PH_SPHERE;  // su1, su2 hold this event's return draw and new |mu_z|
       } else {
-        muz = DM<R>::csign(ct, muz);
+        constexpr uint32_t kLayerBytes = (uint32_t)sizeof(Layer<R>);
         lofs = layer_offset<R>(set, P.first_layer[side]);
-        const Layer<R>& NL = layer_rec<R>(pts, lofs);
-        L_half = NL.half; L_step = NL.step_k; L_abs = NL.absfrac;
-        L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
-        L_kind = NL.kind;
-        muz = DM<R>::csign(glass_entry_cos<R>(ct, P.ent_cmin[side]), muz);
-        zr = side ? NL.half : -NL.half;  // the stack's face: first layer's bottom / last's top
-        s = L_kind == KIND_GLASS ? (R)0 : DM<R>::nlog2(Draw<R>::u(u1)) * L_step;
-        phase = PH_FLIGHT;
+        const R cg = glass_entry_cos<R>(ct, P.ent_cmin[side]);  // the cosine in the first layer
+        if (kSlides && layer_rec<R>(pts, lofs).kind == KIND_SLIDE) {
+          // The first layer is a slide: the packet runs in it between the
+          // ambient face (rA) and the face of the layer behind (rB), both at
+          // the slide's cosine, until it leaves: back out through the
+          // ambient face with probability rB (1 - rA) / (1 - rA rB) (the
+          // interface pass's closed form, from inside), the entrance draw
+          // rescaled to the event u >= Re deciding; the packet then leaves
+          // as a reflected entrance does.  Otherwise it enters the layer
+          // behind with a zero step (CUDAISMLMC.h:230-235).
+          const Layer<R>& SL = layer_rec<R>(pts, lofs);
+          const Iface<R> fa = SL.out[side ? 1 : 0], fb = SL.out[side ? 0 : 1];
+          R cta, ctb;
+          const R rA = fresnel<R>(fa.eta, fa.qq, cg, cta);
+          const R rB = fresnel<R>(fb.eta, fb.qq, cg, ctb);
+          if ((u0 - Re) * ((R)1 - rA * rB) < ((R)1 - Re) * rB * ((R)1 - rA)) {
+            phase = PH_SPHERE;  // su1, su2 hold this event's return draw and new |mu_z|
+          } else {
+            muz = DM<R>::csign(ctb, muz);
+            lofs = side ? lofs - kLayerBytes : lofs + kLayerBytes;
+            const Layer<R>& NL = layer_rec<R>(pts, lofs);
+            L_half = NL.half; L_step = NL.step_k; L_abs = NL.absfrac;
+            L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
+            L_kind = NL.kind;
+            zr = side ? NL.half : -NL.half;  // the face entered: the layer's bottom / top
+            s = (R)0;
+            phase = PH_FLIGHT;
+          }
+        } else {
+          const Layer<R>& NL = layer_rec<R>(pts, lofs);
+          L_half = NL.half; L_step = NL.step_k; L_abs = NL.absfrac;
+          L_ha = NL.hg_a; L_hb = NL.hg_b; L_hc = NL.hg_c; L_hd = NL.hg_d; L_he = NL.hg_e;
+          L_kind = NL.kind;
+          muz = DM<R>::csign(cg, muz);
+          zr = side ? NL.half : -NL.half;  // the stack's face: first layer's bottom / last's top
+          s = L_kind >= KIND_GLASS ? (R)0 : DM<R>::nlog2(Draw<R>::u(u1)) * L_step;
+          phase = PH_FLIGHT;
+        }
       }
       ev += 1;
     }
@@ -1113,6 +1188,17 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
       Ly.mu_t = (R)mu_t;
     }
   }
+  // slides: glass layers between scattering layers or the ambient medium,
+  // whose bounces the kernel takes in closed form (unless the caller asks
+  // for the reference's face-by-face simulation)
+  if (!(flags & ISMC_FLAG_SLIDE_BOUNCES)) {
+    for (int i = 0; i < L; ++i) {
+      const bool lo_ok = i == 0 || cfg->mu_s[i - 1] != 0.0;
+      const bool hi_ok = i == L - 1 || cfg->mu_s[i + 1] != 0.0;
+      if (cfg->mu_s[i] == 0.0 && lo_ok && hi_ok) P->layer[i].kind = KIND_SLIDE;
+    }
+  }
+  // (the faces that lead into a slide are marked below, mut = +inf)
   const double namb = cfg->n_ambient;
   for (int i = 0; i < L; ++i) {
     const bool turbid = cfg->mu_s[i] != 0.0;
@@ -1123,6 +1209,7 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
                                            inside && cfg->mu_s[j] != 0.0, inside && cfg->mu_s[j] == 0.0,
                                            inside ? cfg->mu_t[j] : 1.0);
       if (!inside) P->layer[i].out[dir].mut = (R)-1;
+      else if (turbid && P->layer[j].kind == KIND_SLIDE) P->layer[i].out[dir].mut = (R)INFINITY;
     }
   }
   P->t_tot = (R)z;
@@ -1157,19 +1244,48 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
   }
 }
 
+// Whether a configuration has a slide whose bounces the kernel takes in
+// closed form (compile_point marks the same layers).
+static bool has_slides(const ismc_config* cfg, uint32_t flags) {
+  if (flags & ISMC_FLAG_SLIDE_BOUNCES) return false;
+  const int L = cfg->n_layers;
+  for (int i = 0; i < L; ++i) {
+    const bool lo_ok = i == 0 || cfg->mu_s[i - 1] != 0.0;
+    const bool hi_ok = i == L - 1 || cfg->mu_s[i + 1] != 0.0;
+    if (cfg->mu_s[i] == 0.0 && lo_ok && hi_ok) return true;
+  }
+  return false;
+}
+
+template <typename R>
+static const void* transport_fn(bool per_packet, bool slides) {
+  if (per_packet)
+    return slides ? (const void*)transport_kernel<R, true, true> : (const void*)transport_kernel<R, true, false>;
+  return slides ? (const void*)transport_kernel<R, false, true> : (const void*)transport_kernel<R, false, false>;
+}
+
+template <typename R>
+static void launch_transport(bool per_packet, bool slides, int grid, hipStream_t st, const Args& a) {
+  if (per_packet) {
+    if (slides) transport_kernel<R, true, true><<<grid, kBlock, 0, st>>>(a);
+    else transport_kernel<R, true, false><<<grid, kBlock, 0, st>>>(a);
+  } else {
+    if (slides) transport_kernel<R, false, true><<<grid, kBlock, 0, st>>>(a);
+    else transport_kernel<R, false, false><<<grid, kBlock, 0, st>>>(a);
+  }
+}
+
 // Launch shape of a transport launch: workgroups per CU (the occupancy
 // limit unless the caller asks for a number) and packets per queue chunk.
 template <typename R>
 static int launch_shape(ismc_ctx* c, const uint64_t* npk, int32_t n_points, const ismc_tune* tune,
-                        bool per_packet, int* per_cu_out, int* chunk_out) {
+                        bool per_packet, bool slides, int* per_cu_out, int* chunk_out) {
   const int bpc = (tune && tune->blocks_per_cu > 0) ? tune->blocks_per_cu : c->blocks_per_cu;
   // persistent grid: as many blocks as stay resident (occupancy), unless the
   // caller asks for a given number per CU
   int resident = 0;
-  if (per_packet)
-    HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, true>, kBlock, 0));
-  else
-    HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, false>, kBlock, 0));
+  HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_fn<R>(per_packet, slides),
+                                                           kBlock, 0));
   const int per_cu = (tune && tune->blocks_per_cu > 0) ? bpc : std::max(1, resident);
   // packets per queue chunk: the caller's, else a size fitted to the
   // launch.  Each chunk costs a queue atomic, a point-table copy and a
@@ -1270,7 +1386,9 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
                       uint64_t seed, uint32_t flags, const ismc_tune* tune, ismc_tally* out,
                       double* per_packet_host) {
   int per_cu = 1, chunk = 1, rc;
-  if ((rc = launch_shape<R>(c, npk, n_points, tune, per_packet_host != nullptr, &per_cu, &chunk)))
+  bool slides = false;
+  for (int p = 0; p < n_points && !slides; ++p) slides = has_slides(&cfgs[p], flags);
+  if ((rc = launch_shape<R>(c, npk, n_points, tune, per_packet_host != nullptr, slides, &per_cu, &chunk)))
     return rc;
   std::vector<Range> ranges((size_t)n_points);
   uint64_t total = 0, n_chunks = 0;
@@ -1310,10 +1428,7 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
         (unsigned long long*)c->slots.p, nslot, (uint32_t*)c->counter.p);
     HIPCHECK(c, hipGetLastError());
     if (timed) HIPCHECK(c, hipEventRecord(c->e0, c->stream));
-    if (per_packet_host)
-      transport_kernel<R, true><<<grid, kBlock, 0, c->stream>>>(a);
-    else
-      transport_kernel<R, false><<<grid, kBlock, 0, c->stream>>>(a);
+    launch_transport<R>(per_packet_host != nullptr, slides, grid, c->stream, a);
     HIPCHECK(c, hipGetLastError());
     if (timed) HIPCHECK(c, hipEventRecord(c->e1, c->stream));
     HIPCHECK(c, hipMemcpyAsync(hs, c->slots.p, sbytes, hipMemcpyDeviceToHost, c->stream));
@@ -1327,7 +1442,7 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
         (uint64_t)a.idle_lanes, (uint64_t)a.bound_lanes,
         (uint64_t)(uintptr_t)c->points.p, (uint64_t)(uintptr_t)c->ranges.p,
         (uint64_t)(uintptr_t)c->slots.p, (uint64_t)(uintptr_t)c->counter.p,
-        (uint64_t)(uintptr_t)c->h_stage, pbytes, cbytes, sbytes, seed};
+        (uint64_t)(uintptr_t)c->h_stage, pbytes, cbytes, sbytes, seed, (uint64_t)slides};
     for (auto& g : c->graphs)
       if (g.key == key) hit = &g;
     // a shape is captured the second time it comes: one-off shapes (a
@@ -1442,7 +1557,9 @@ static int run_points_static(ismc_ctx* c, const ismc_config* cfgs, const uint64_
     return set_err(c, ISMC_ERANGE, "at most %s%lld configurations per no-allocation call", "",
                    kStaticMaxPoints);
   int per_cu = 1, chunk = 1, rc;
-  if ((rc = launch_shape<R>(c, npk, n_points, tune, false, &per_cu, &chunk))) return rc;
+  bool slides = false;
+  for (int p = 0; p < n_points && !slides; ++p) slides = has_slides(&cfgs[p], flags);
+  if ((rc = launch_shape<R>(c, npk, n_points, tune, false, slides, &per_cu, &chunk))) return rc;
   // every range is checked before any device work
   uint64_t total = 0, n_chunks = 0;
   for (int p = 0; p < n_points; ++p) {
@@ -1479,7 +1596,7 @@ static int run_points_static(ismc_ctx* c, const ismc_config* cfgs, const uint64_
   zero_u64_kernel<<<(unsigned)((nslot + 255) / 256), 256, 0, c->stream>>>(d_slots, nslot, d_counter);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->e0, c->stream));
-  transport_kernel<R, false><<<grid, kBlock, 0, c->stream>>>(a);
+  launch_transport<R>(false, slides, grid, c->stream, a);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->e1, c->stream));
   // tallies back through the same buffer

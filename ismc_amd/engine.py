@@ -124,37 +124,42 @@ class Engine:
                 "chunks": nch.value}
 
     @staticmethod
-    def _flags(fp64, cuda_roulette):
-        return (_lib.FLAG_FP64 if fp64 else 0) | (_lib.FLAG_CUDA_ROULETTE if cuda_roulette else 0)
+    def _flags(fp64, cuda_roulette, slide_bounces=False):
+        return ((_lib.FLAG_FP64 if fp64 else 0) | (_lib.FLAG_CUDA_ROULETTE if cuda_roulette else 0) |
+                (_lib.FLAG_SLIDE_BOUNCES if slide_bounces else 0))
 
     def simulate(self, cfg: SampleConfig, n_packets: int, seed: int = 1234,
                  first_packet_id: int = 0, stream: int = 0, fp64: bool = False,
-                 cuda_roulette: bool = False) -> Result:
+                 cuda_roulette: bool = False, slide_bounces: bool = False) -> Result:
+        """One configuration, n_packets packets.  slide_bounces simulates a
+        glass slide's internal bounces face by face as the reference does
+        (by default they are taken in closed form: same tallies in
+        distribution, fewer boundary events)."""
         c = cfg.to_c()
         t = _lib.Tally()
         rc = self._L.ismc_simulate(self._ctx, ctypes.byref(c), int(n_packets), int(seed),
                                    int(first_packet_id), int(stream),
-                                   self._flags(fp64, cuda_roulette), ctypes.byref(t))
+                                   self._flags(fp64, cuda_roulette, slide_bounces), ctypes.byref(t))
         _lib.check(self._ctx, rc, "ismc_simulate")
         return _result(t, self.last_launch()["kernel_ms"])
 
     def simulate_packets(self, cfg: SampleConfig, n_packets: int, seed: int = 1234,
                          first_packet_id: int = 0, stream: int = 0, fp64: bool = False,
-                         cuda_roulette: bool = False):
+                         cuda_roulette: bool = False, slide_bounces: bool = False):
         """Result plus an (n, 6) array of per-packet {A, R_d, R_s, T_d, T_u, events}."""
         c = cfg.to_c()
         t = _lib.Tally()
         out = np.zeros((int(n_packets), 6), dtype=np.float64)
         rc = self._L.ismc_simulate_packets(self._ctx, ctypes.byref(c), int(n_packets), int(seed),
                                            int(first_packet_id), int(stream),
-                                           self._flags(fp64, cuda_roulette),
+                                           self._flags(fp64, cuda_roulette, slide_bounces),
                                            out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(t))
         _lib.check(self._ctx, rc, "ismc_simulate_packets")
         return _result(t, self.last_launch()["kernel_ms"]), out
 
     def simulate_batch(self, cfgs: Sequence[SampleConfig], n_packets, seed: int = 1234,
                        first_packet_ids=None, streams=None, fp64: bool = False,
-                       cuda_roulette: bool = False, blocks_per_cu: int = 0,
+                       cuda_roulette: bool = False, slide_bounces: bool = False, blocks_per_cu: int = 0,
                        chunk_packets: int = 0, use_graph: bool = False, autotune: bool = False,
                        idle_lanes: int = 0, bound_lanes: int = 0):
         """Many configurations, uneven packet counts, one persistent launch.
@@ -190,7 +195,7 @@ class Engine:
             self._ctx, arr.ctypes.data_as(ctypes.POINTER(_lib.Config)), npk.ctypes.data_as(P64),
             first.ctypes.data_as(P64) if first is not None else None,
             st.ctypes.data_as(P32) if st is not None else None,
-            npts, int(seed), self._flags(fp64, cuda_roulette), ctypes.byref(tune),
+            npts, int(seed), self._flags(fp64, cuda_roulette, slide_bounces), ctypes.byref(tune),
             outs.ctypes.data_as(ctypes.POINTER(_lib.Tally)))
         _lib.check(self._ctx, rc, "ismc_simulate_batch")
         return _results(outs, self.last_launch()["kernel_ms"])

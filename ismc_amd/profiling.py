@@ -31,7 +31,7 @@ import json
 import os
 import sys
 
-KERNEL = "transport_kernel<float, false>"
+KERNEL = "transport_kernel<float, false, true>"  # the slab between slides has slides
 COPY = "copy_probe_kernel"
 
 

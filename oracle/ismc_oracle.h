@@ -46,6 +46,9 @@
 #define ORACLE_FP32      1u   /* do the transport arithmetic in float       */
 #define ORACLE_MIRROR    2u   /* ISMC.py single-slab mirror propagator       */
 #define ORACLE_CUDA_ROULETTE 4u /* CUDAISMLMC.h:275 integer 1/m == 0: never survive */
+#define ORACLE_ETALON    8u   /* a slide's bounces in closed form (one event);
+                                 the default restates the reference's
+                                 bounce-by-bounce simulation */
 
 typedef struct {
     int32_t present;

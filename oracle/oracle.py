@@ -30,6 +30,7 @@ MAX_LAYERS = 16
 FP32 = 1
 MIRROR = 2
 CUDA_ROULETTE = 4
+ETALON = 8
 TALLIES = ("A", "R_d", "R_s", "T_d", "T_u")
 EVENTS = ("interactions", "boundary_hits", "entrances", "sphere_decisions", "reinjections")
 

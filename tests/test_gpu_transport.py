@@ -106,22 +106,32 @@ def test_fp32_spin_is_the_exact_rotation(engine):
     assert np.max(np.abs(out[near, 0] - np.sign(sp[near, 0]) * ct[near])) <= np.sqrt(2e-5) + 1e-5
 
 
+# the kernel's default (a slide's bounces in closed form) against the
+# oracle's ORACLE_ETALON, and the reference's face-by-face slide bounces
+# (slide_bounces) against the oracle's default restatement
+SLIDE_MODES = [(False, O.ETALON), (True, 0)]
+
+
+@pytest.mark.parametrize("slide_bounces,oflags", SLIDE_MODES, ids=["closed_form", "bounces"])
 @pytest.mark.parametrize("name,cfg", CASES, ids=[c[0] for c in CASES])
-def test_fp64_kernel_matches_oracle_per_packet(engine, name, cfg):
+def test_fp64_kernel_matches_oracle_per_packet(engine, name, cfg, slide_bounces, oflags):
     n = 4000
-    _, g = engine.simulate_packets(cfg, n, seed=77, first_packet_id=1000, stream=3, fp64=True)
-    o = O.simulate(spec_of(cfg), n, seed=77, first_pid=1000, stream=3, per_packet=True)["per_packet"]
+    _, g = engine.simulate_packets(cfg, n, seed=77, first_packet_id=1000, stream=3, fp64=True,
+                                   slide_bounces=slide_bounces)
+    o = O.simulate(spec_of(cfg), n, seed=77, first_pid=1000, stream=3, per_packet=True,
+                   flags=oflags)["per_packet"]
     same = np.all(np.abs(g[:, :5] - o[:, :5]) < 1e-6, axis=1) & (g[:, 5] == o[:, 5])
     print(name, "identical packets: %.5f" % same.mean())
     assert same.mean() >= 0.999
     assert np.abs(g[:, :5].sum(0) - o[:, :5].sum(0)).max() < 1e-3 * n ** 0.5
 
 
+@pytest.mark.parametrize("slide_bounces,oflags", SLIDE_MODES, ids=["closed_form", "bounces"])
 @pytest.mark.parametrize("name,cfg", CASES, ids=[c[0] for c in CASES])
-def test_fp32_kernel_tracks_oracle(engine, name, cfg):
+def test_fp32_kernel_tracks_oracle(engine, name, cfg, slide_bounces, oflags):
     n = 4000
-    _, g = engine.simulate_packets(cfg, n, seed=5, fp64=False)
-    o = O.simulate(spec_of(cfg), n, seed=5, per_packet=True, flags=O.FP32)["per_packet"]
+    _, g = engine.simulate_packets(cfg, n, seed=5, fp64=False, slide_bounces=slide_bounces)
+    o = O.simulate(spec_of(cfg), n, seed=5, per_packet=True, flags=O.FP32 | oflags)["per_packet"]
     same = np.all(np.abs(g[:, :5] - o[:, :5]) < 1e-4, axis=1)
     print(name, "FP32 packets within 1e-4 of the FP32 oracle: %.4f" % same.mean())
     # measured 0.9990-1.0000 on these cases: a packet parts from the oracle's
@@ -180,6 +190,46 @@ def test_large_run_properties(engine):
     # the roulette kills inside the sample (a few in 2^27 here)
     assert 0 <= ev["entrances"] - ev["sphere_decisions"] <= 1e-6 * (1 << 27)
     assert ev["reinjections"] < ev["entrances"]
+
+
+def _slide_stacks():
+    yield "slab_between_slides", slab_between_slides()
+    c = slab_between_slides()
+    c.layers = [Layer(1.40, 0.9, 0.05, 1.0, 100.0), Layer.glass(1.52, 0.1), Layer(1.33, 0.8, 0.05, 0.5, 50.0)]
+    yield "sample_slide_sample", c
+    c = slab_between_slides()
+    c.layers = [Layer(1.60, 0.7, 0.1, 0.3, 30.0), Layer.glass(1.45, 0.2)]
+    yield "dense_sample_slide", c
+
+
+@pytest.mark.parametrize("name,cfg", list(_slide_stacks()), ids=[c[0] for c in _slide_stacks()])
+def test_slide_closed_form_equals_bounces(engine, name, cfg):
+    # A slide's bounces taken in closed form (the default) against the
+    # reference's face-by-face simulation (slide_bounces), 2^26 packets
+    # each on independent streams: every tally within 4 combined SE (15
+    # comparisons over the three stacks), the same interaction and entrance
+    # counts per packet within their noise, and fewer boundary events.
+    n = 1 << 26
+    a = engine.simulate(cfg, n, seed=61)
+    b = engine.simulate(cfg, n, seed=62, slide_bounces=True)
+    z = zscores(vec(a.mean), vec(a.se), vec(b.mean), vec(b.se))
+    print(name, "z", np.round(z, 2), "hits/packet %.3f -> %.3f" % (b.events["boundary_hits"] / n,
+                                                                   a.events["boundary_hits"] / n))
+    assert np.all(np.abs(z) < 4.0), np.round(z, 2)
+    assert a.events["boundary_hits"] < 0.8 * b.events["boundary_hits"]
+    for k in ("interactions", "entrances"):
+        assert abs(a.events[k] - b.events[k]) / b.events[k] < 2e-3, (k, a.events[k], b.events[k])
+
+
+def test_closed_form_changes_nothing_without_slides(engine):
+    # no slide in the stack (no glass, or glass next to glass): the flag has
+    # nothing to change, tallies and events are bitwise the same
+    c = slab_between_slides()
+    c.layers = [Layer.glass(1.52, 0.1), Layer.glass(1.45, 0.1), Layer(1.40, 0.9, 0.1, 1.0, 100.0)]
+    for cfg in (three_layer_sample(), c):
+        a = engine.simulate(cfg, 1 << 22, seed=9)
+        b = engine.simulate(cfg, 1 << 22, seed=9, slide_bounces=True)
+        assert a.sum == b.sum and a.sumsq == b.sumsq and a.events == b.events
 
 
 def test_full_size_step_is_additive_over_packet_ranges(engine):

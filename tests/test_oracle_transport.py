@@ -105,3 +105,57 @@ def test_grazing_glass_entry_cannot_trap_a_packet():
     assert abs(r["per_packet"][0, :5].sum() - 1) < 1e-6 or r["per_packet"][0, :5].sum() < 1e-3
     big = O.simulate(spec, 2_000_000, seed=2024, flags=O.FP32, threads=8)
     assert big["n_aborted"] == 0
+
+
+SPHERE = {"present": 1, "pw": .98, "fs": 0.003921629, "fp": 0.003921629, "fd": 0.000977518}
+GLASS = {"n": 1.52, "g": 0, "t": .1, "mu_a": 0, "mu_s": 0}
+SAMPLE = {"n": 1.40, "g": .9, "t": .1, "mu_a": 1, "mu_s": 100}
+
+
+def _stack(*layers):
+    return {"layers": list(layers), "R": dict(SPHERE), "T": dict(SPHERE)}
+
+
+@pytest.mark.parametrize("name,spec", [
+    ("slide|sample|slide", _stack(GLASS, SAMPLE, GLASS)),
+    ("sample|slide|sample", _stack(dict(SAMPLE, t=.05), GLASS,
+                                   {"n": 1.33, "g": .8, "t": .05, "mu_a": .5, "mu_s": 50})),
+    ("dense sample|slide", _stack({"n": 1.60, "g": .7, "t": .1, "mu_a": .3, "mu_s": 30},
+                                  dict(GLASS, n=1.45, t=.2))),
+])
+def test_slide_closed_form_matches_bounces(name, spec):
+    # ORACLE_ETALON takes a slide's bounces as the geometric series that the
+    # face-by-face simulation (the reference's, restated by default) sums
+    # one bounce at a time: the tallies agree within 4 combined SE (15
+    # comparisons) on independent streams, the interactions per packet too,
+    # and the boundary events fall.
+    N = 1_000_000
+    a = O.simulate(spec, N, seed=31, threads=8)
+    b = O.simulate(spec, N, seed=32, threads=8, flags=O.ETALON)
+    for k in a["mean"]:
+        se = math.hypot(a["se"][k], b["se"][k])
+        assert abs(a["mean"][k] - b["mean"][k]) < 4 * se + 1e-12, (name, k, a["mean"][k], b["mean"][k], se)
+    ia, ib = a["events"]["interactions"] / N, b["events"]["interactions"] / N
+    assert abs(ia - ib) < 0.01 * ia, (ia, ib)
+    assert b["events"]["boundary_hits"] < 0.8 * a["events"]["boundary_hits"]
+
+
+def test_slide_closed_form_changes_nothing_without_slides():
+    # no slide (no glass, or glass next to glass): per packet identical
+    for spec in (_stack(SAMPLE), _stack(GLASS, dict(GLASS, n=1.45), SAMPLE),
+                 _stack(dict(SAMPLE, t=.03), {"n": 1.44, "g": .8, "t": .05, "mu_a": 2, "mu_s": 80})):
+        a = O.simulate(spec, 3000, seed=4, per_packet=True)["per_packet"]
+        b = O.simulate(spec, 3000, seed=4, per_packet=True, flags=O.ETALON)["per_packet"]
+        assert np.array_equal(a, b)
+
+
+def test_slide_closed_form_single_packet_paths():
+    # slide|sample|slide with the closed form: the boundary events are the
+    # sample's crossings into a slide only (the glass bounces are gone), and
+    # each packet's weight is conserved up to roulette (which moves at most
+    # 1e-4 of it, times m - 1 = 9 on survival)
+    spec = _stack(GLASS, SAMPLE, GLASS)
+    r = O.simulate(spec, 20000, seed=6, per_packet=True, flags=O.ETALON)
+    pp = r["per_packet"]
+    assert np.all(np.abs(pp[:, :5].sum(1) - 1) < 1e-3)
+    assert r["events"]["boundary_hits"] / 20000 < 2.6
