@@ -103,9 +103,10 @@ typedef struct {
     int32_t use_graph;       /* replay through a captured hipGraph when shapes repeat */
     int32_t idle_lanes;      /* lanes of a wave waiting for an entrance, sphere decision
                                 or new packet before the wave serves them (1..64;
-                                default 10) */
+                                default 10, 12 when a point has a glass slide) */
     int32_t bound_lanes;     /* lanes of a wave waiting at an interface before the wave
-                                serves them (1..64; default 12) */
+                                serves them (1..64; default 12, 16 when a point has a
+                                glass slide) */
     int32_t _reserved;
 } ismc_tune;
 

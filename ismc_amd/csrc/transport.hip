@@ -1009,11 +1009,14 @@ struct ismc_ctx {
   int64_t last_chunks = 0;
   int blocks_per_cu = 8;
   int chunk_packets = 4096;
-  // measured on MI355X (slab between slides / three layers, packets/s):
-  // 10 / 12 beat 8 / 16 by 0.6 % / 0.5 %; 8 / 12 and 12 / 12 are within
-  // 0.4 % on the slab, 20 bound lanes and 6 idle lanes are slower
+  // measured on MI355X (packets/s): without slides (three layers, one
+  // layer with mu_s 10) 10 / 12 beat 8 / 16 and 12 / 16 by 0.5-2 %; with
+  // slides (the slab between them, whose slides the kernel takes in closed
+  // form: 2.3 interface hits per packet) 12 / 16 beats 10 / 12 by 1.5 %
   int idle_lanes = 10;
   int bound_lanes = 12;
+  int idle_lanes_slides = 12;
+  int bound_lanes_slides = 16;
   // captured launch sequences, replayed when a call repeats a shape
   struct Graph {
     std::vector<uint64_t> key;
@@ -1339,7 +1342,7 @@ static int point_range(ismc_ctx* c, int p, const uint64_t* npk, const uint64_t* 
 }
 
 // Kernel arguments of a launch whose buffers are given.
-static Args make_args(ismc_ctx* c, const ismc_tune* tune, void* points, void* ranges, void* slots,
+static Args make_args(ismc_ctx* c, const ismc_tune* tune, bool slides, void* points, void* ranges, void* slots,
                       void* counter, double* per_packet, int32_t n_points, uint64_t n_chunks,
                       int chunk, uint64_t seed) {
   Args a{};
@@ -1353,10 +1356,10 @@ static Args make_args(ismc_ctx* c, const ismc_tune* tune, void* points, void* ra
   a.chunk_counter = (uint32_t*)counter;
   a.slots = (unsigned long long*)slots;
   a.per_packet = per_packet;
-  a.idle_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->idle_lanes > 0) ? tune->idle_lanes
-                                                                                 : c->idle_lanes));
-  a.bound_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->bound_lanes > 0) ? tune->bound_lanes
-                                                                                   : c->bound_lanes));
+  const int idle0 = slides ? c->idle_lanes_slides : c->idle_lanes;
+  const int bound0 = slides ? c->bound_lanes_slides : c->bound_lanes;
+  a.idle_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->idle_lanes > 0) ? tune->idle_lanes : idle0));
+  a.bound_lanes = (uint32_t)std::min(64, std::max(1, (tune && tune->bound_lanes > 0) ? tune->bound_lanes : bound0));
   return a;
 }
 
@@ -1413,7 +1416,7 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
     if ((rc = ensure_dev(c, c->per_packet, sizeof(double) * 6 * std::max<uint64_t>(total, 1))))
       return rc;
   }
-  const Args a = make_args(c, tune, c->points.p, c->ranges.p, c->slots.p, c->counter.p,
+  const Args a = make_args(c, tune, slides, c->points.p, c->ranges.p, c->slots.p, c->counter.p,
                            per_packet_host ? (double*)c->per_packet.p : nullptr, n_points, n_chunks,
                            chunk, seed);
   const int grid = grid_for(c, per_cu, total);
@@ -1590,7 +1593,7 @@ static int run_points_static(ismc_ctx* c, const ismc_config* cfgs, const uint64_
     HIPCHECK(c, hipStreamSynchronize(c->stream));
   }
   const size_t nslot = (size_t)kSlotWords * n_points;
-  const Args a = make_args(c, tune, d_points, d_ranges, d_slots, d_counter, nullptr, n_points, n_chunks,
+  const Args a = make_args(c, tune, slides, d_points, d_ranges, d_slots, d_counter, nullptr, n_points, n_chunks,
                            chunk, seed);
   const int grid = grid_for(c, per_cu, total);
   zero_u64_kernel<<<(unsigned)((nslot + 255) / 256), 256, 0, c->stream>>>(d_slots, nslot, d_counter);

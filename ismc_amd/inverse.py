@@ -617,6 +617,18 @@ def run(sur, count=4096, meas_packets=1_000_000, refine_packets=100_000, check_p
     return out
 
 
+def rate_ci(secs, n_boot=4000, seed=0):
+    """95 % bootstrap interval of a rate len(secs) / sum(secs) measured
+    over independent solves (resampling the solves; the per-solve time is
+    heavy-tailed, so the interval is asymmetric)."""
+    secs = np.asarray(secs, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, len(secs), size=(n_boot, len(secs)))
+    rates = len(secs) / secs[idx].sum(axis=1)
+    lo, hi = np.percentile(rates, [2.5, 97.5])
+    return [float(lo), float(hi)]
+
+
 def _plain_stats(probs, meas, cost_truth, runs, count, timed, packets, tol, check_packets, eng, log):
     """Accuracy of the plain solver over the first `count` measurements (run
     in lock step, plain_batch) and its solves/s from the first `timed` of
@@ -645,6 +657,7 @@ def _plain_stats(probs, meas, cost_truth, runs, count, timed, packets, tol, chec
             # total time and the spread is reported beside it
             "seconds_per_solve": ({"median": float(np.median(secs)), "p90": float(np.percentile(secs, 90)),
                                    "max": float(np.max(secs))} if timed else None),
+            "solves_per_s_ci95": rate_ci(secs) if timed else None,
             "mean_evals_timed": float(np.mean(sevals)) if timed else None,
             "mean_evals": float(np.mean(nev)), "retries": int(retries.sum()),
             "sequential_equals_lockstep": same, "lockstep_s": t_batch, "packets_per_eval": packets,
