@@ -345,7 +345,10 @@ def main():
         "config": {"workload": "single-layer slab between glass slides, dual integrating spheres",
                    "packets_per_step": N, "layers": "glass 1.52/0.1 | mu_a 1, mu_s 100, g 0.9, "
                    "n 1.40, 0.1 cm | glass 1.52/0.1", "spheres": "R=4 cm, ports 0.5/0.5/0.25 cm, pw 0.98",
-                   "roulette": "1e-4, 1 in 10", "seed": 1234, "grid_blocks": launch["grid"],
+                   "roulette": "1e-4, 1 in 10", "seed": 1234,
+                   "slides": "bounces inside each slide in closed form (one boundary event per crossing; "
+                             "ISMC_FLAG_SLIDE_BOUNCES gives the reference's face-by-face events, same tallies)",
+                   "grid_blocks": launch["grid"],
                    "block_threads": launch["block"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tslot/s",
                      "frac": achieved / peak, "traffic": traffic,

@@ -114,8 +114,8 @@ class AutoTuner:
             table.append(t)
             return t
 
-        # start from the library's defaults (occupancy-limited grid, 10 idle
-        # lanes, 12 bound lanes)
+        # start from the library's defaults for launches without a glass slide
+        # (occupancy-limited grid, 10 idle lanes, 12 bound lanes)
         best = {"blocks_per_cu": 0, "chunk_packets": chunks[0],
                 "idle_lanes": 10 if 10 in self.idle else self.idle[0],
                 "bound_lanes": 12 if 12 in self.bound else self.bound[0]}
