@@ -1247,6 +1247,38 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
   }
 }
 
+// The order in which a launch's points hand out their chunks: costliest
+// packets first.  Waves take chunks in point order, so a chunk of a
+// long-history point that starts last holds the whole launch until it
+// ends; started first, it overlaps the rest.  Measured on MI355X: 1024
+// points log-uniform in mu_a 0.01-10 and mu_s 1-1000 /cm (0.1 cm, g 0.8),
+// 2^17 packets each, 13-15 % less kernel time than in input order; a 32 x 32
+// lookup grid at 2^16 packets per point, 19-27 % less.  Only the order
+// matters (each point keeps its own Philox stream), and the estimate
+// (a diffusion-like count of interactions per packet: mu_s t (1 + mu_s' t)
+// damped by absorption, summed over the scattering layers) needs only to
+// rank points roughly.  Tallies do not depend on the order.
+static double cost_estimate(const ismc_config* cfg) {
+  double e = 1.0;
+  for (int i = 0; i < cfg->n_layers; ++i) {
+    if (cfg->mu_s[i] == 0.0) continue;
+    const double tau = cfg->mu_s[i] * cfg->t[i];
+    const double tau_r = tau * (1.0 - cfg->g[i]);
+    e += tau * (1.0 + tau_r) / (1.0 + 10.0 * cfg->mu_a[i] * cfg->t[i]);
+  }
+  return e;
+}
+
+static std::vector<int> launch_order(const ismc_config* cfgs, int32_t n_points, bool keep) {
+  std::vector<int> order((size_t)n_points);
+  for (int p = 0; p < n_points; ++p) order[(size_t)p] = p;
+  if (keep || n_points < 2) return order;
+  std::vector<double> est((size_t)n_points);
+  for (int p = 0; p < n_points; ++p) est[(size_t)p] = cost_estimate(&cfgs[p]);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return est[(size_t)a] > est[(size_t)b]; });
+  return order;
+}
+
 // Whether a configuration has a slide whose bounces the kernel takes in
 // closed form (compile_point marks the same layers).
 static bool has_slides(const ismc_config* cfg, uint32_t flags) {
@@ -1393,10 +1425,13 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   for (int p = 0; p < n_points && !slides; ++p) slides = has_slides(&cfgs[p], flags);
   if ((rc = launch_shape<R>(c, npk, n_points, tune, per_packet_host != nullptr, slides, &per_cu, &chunk)))
     return rc;
+  // points in launch order (costliest first; a per-packet launch keeps its
+  // order, its output rows follow the points)
+  const std::vector<int> order = launch_order(cfgs, n_points, per_packet_host != nullptr);
   std::vector<Range> ranges((size_t)n_points);
   uint64_t total = 0, n_chunks = 0;
-  for (int p = 0; p < n_points; ++p)
-    if ((rc = point_range(c, p, npk, first, streams, chunk, &n_chunks, &total, &ranges[(size_t)p])))
+  for (int k = 0; k < n_points; ++k)
+    if ((rc = point_range(c, order[(size_t)k], npk, first, streams, chunk, &n_chunks, &total, &ranges[(size_t)k])))
       return rc;
   if (n_chunks > 0xffffffffull) return set_err(c, ISMC_ERANGE, "too many chunks%s");
   const size_t pbytes = sizeof(Point<R>) * (size_t)n_points;
@@ -1409,7 +1444,7 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   if ((rc = ensure_stage(c, pbytes + cbytes + sbytes))) return rc;
   char* st = (char*)c->h_stage;
   Point<R>* hp = reinterpret_cast<Point<R>*>(st);
-  for (int p = 0; p < n_points; ++p) compile_point<R>(&cfgs[p], flags, &hp[p]);
+  for (int k = 0; k < n_points; ++k) compile_point<R>(&cfgs[order[(size_t)k]], flags, &hp[k]);
   memcpy(st + pbytes, ranges.data(), cbytes);
   const size_t nslot = (size_t)kSlotWords * n_points;
   if (per_packet_host) {
@@ -1502,7 +1537,10 @@ static int run_points(ismc_ctx* c, const ismc_config* cfgs, const uint64_t* npk,
   HIPCHECK(c, hipEventElapsedTime(&c->last_ms, c->e0, c->e1));
   c->last_grid = grid;
   c->last_chunks = (int64_t)n_chunks;
-  for (int p = 0; p < n_points; ++p) slot_to_tally(hs + (size_t)p * kSlotWords, npk[p], &out[p]);
+  for (int k = 0; k < n_points; ++k) {
+    const int p = order[(size_t)k];
+    slot_to_tally(hs + (size_t)k * kSlotWords, npk[p], &out[p]);
+  }
   return 0;
 }
 
