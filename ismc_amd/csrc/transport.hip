@@ -221,6 +221,7 @@ template <> struct DM<float> {
   static __device__ __forceinline__ float nlog2(float u) { return -__builtin_amdgcn_logf(u); }
   static __device__ __forceinline__ float cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
   static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
+  static __device__ __forceinline__ float max0(float x) { return __builtin_fmaxf(x, 0.0f); }
   static __device__ __forceinline__ float csign(float a, float b) { return __builtin_copysignf(a, b); }
   // round(v 2^32) for 0 <= v < 2^32 as the integer part (a truncating
   // convert) and round(fract(v) 2^32), which is below 2^32 - 255 for an FP32
@@ -239,6 +240,7 @@ template <> struct DM<double> {
   static __device__ __forceinline__ double nlog2(double u) { return -::log2(u); }
   static __device__ __forceinline__ double cos2pi(double u) { return ::cospi(2.0 * u); }
   static __device__ __forceinline__ double abs(double x) { return __builtin_fabs(x); }
+  static __device__ __forceinline__ double max0(double x) { return __builtin_fmax(x, 0.0); }
   static __device__ __forceinline__ double csign(double a, double b) { return __builtin_copysign(a, b); }
   static __device__ __forceinline__ unsigned long long fx(double v) {
     return (unsigned long long)__builtin_rint(v * 4294967296.0);
@@ -730,8 +732,15 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       const R d = dz * DM<R>::rcp(muz);
       zr = zb;
       // the step left (in a slide s is never read, and its transmission
-      // into the next layer multiplies it by f.mut = 0)
-      s = s - d;
+      // into the next layer multiplies it by f.mut = 0), at least 0: a hop
+      // that ended within rounding of the face can leave s - d (d from an
+      // approximate reciprocal) a few ulps below 0, and a reflected packet
+      // with a negative step would hit again at once, be taken to the far
+      // face and, under total internal reflection at both faces, bounce
+      // there until the event limit.  The reference subtracts d only when
+      // s >= d (MLMC.py:100-119); a step of 0 is its s == d case, an
+      // interaction on the face.
+      s = DM<R>::max0(s - d);
       const Iface<R> f = layer_rec<R>(pts, lofs).out[up ? 1 : 0];
       const bool outside = f.mut < (R)0;  // no layer beyond (from the host)
       R ct;

@@ -293,6 +293,28 @@ def test_event_limit_drops_runaway_packets(engine, fp64):
     assert abs(k1 - k2) <= 4 * math.sqrt(k1 + k2) + 1
 
 
+def test_no_packet_trapped_at_a_face(engine):
+    # A bare sample (n 1.40, mu_a 1, mu_s 10 /cm, g 0.8, 0.1 cm, dual
+    # spheres; a measurement set-up of the inverse solver), FP32, seed 7:
+    # among its first 2^28 packets one hop ends within rounding of a face.
+    # The step left after that hit is 0, not a few ulps below it; with a
+    # negative step the reflected packet hit again at once, was taken to the
+    # far face and, under total internal reflection at both faces, bounced
+    # there until the event limit (2^22 boundary hits, the launch held for
+    # 1.8 s instead of 8 ms).  Now every packet leaves: nothing is dropped
+    # and the boundary hits per packet are the ordinary 1.84 (the trapped
+    # packet alone added 0.016).
+    from ismc_amd import inverse
+    cfg = inverse.problem(0.5, 0.5).configs(1.0, 2.0)[0][0]
+    n = 1 << 28
+    r = engine.simulate(cfg, n, seed=7)
+    hits = r.events["boundary_hits"] / n
+    print("aborted", r.n_aborted, "boundary hits per packet", hits, "kernel ms", r.kernel_ms)
+    assert r.n_aborted == 0
+    assert 1.83 < hits < 1.85
+    assert abs(sum(r.mean.values()) - 1.0) < 1e-4
+
+
 @pytest.mark.parametrize("name,cfg", [("slab_between_slides", slab_between_slides()),
                                       ("three_layer", three_layer_sample())])
 def test_fp32_vs_fp64_kernel_deviation(engine, name, cfg):
