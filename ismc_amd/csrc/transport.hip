@@ -124,10 +124,18 @@ enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2, KIND_SLIDE = 3 };
 // finite precision 1 - mu^2 can round so that the way back reads as total
 // internal reflection; with TIR at the far face too (glass->air) the packet
 // would bounce in the non-absorbing slide for ever (the FP32 reference path
-// has no exit from that loop).  The refracted direction is therefore kept
-// at least kGlassGuard (relative, in sin^2) inside the critical cone of the
-// face it came through; this moves only packets within ~sqrt(kGlassGuard)
-// of grazing incidence.
+// has no exit from that loop).  In a run of adjacent glass layers the
+// rounding also accumulates: n^2 (1 - mu^2) is kept only to the last bit at
+// each glass-glass crossing, and where the two directions' index ratios
+// round apart it drifts the same way at every round trip, until neither
+// face that bounds the run lets the packet out.  The refracted direction in
+// a glass layer is therefore kept at least GlassGuard (relative, in sin^2)
+// inside the critical cone of the denser of the two media that bound its
+// run of glass layers (scattering layers or the ambient medium), through
+// which any packet in the run can then leave.  For a slide entered from a
+// scattering layer at least as dense as the medium beyond the slide this is
+// the cone of the face the packet came through; it moves only packets
+// within ~sqrt(GlassGuard) of grazing incidence.
 template <typename R> struct GlassGuard;
 template <> struct GlassGuard<float> { static constexpr float value = 9.5367431640625e-07f; };  // 2^-20
 template <> struct GlassGuard<double> { static constexpr double value = 9.094947017729282e-13; };  // 2^-40
@@ -148,8 +156,9 @@ template <typename R>
 struct alignas(16) Iface {
   R eta;   // n_here / n_there (Snell)
   R qq;    // ((n_there - n_here) / (n_there + n_here))^2, normal incidence
-  R cmin;  // entering a glass slide from a less dense layer: the least |mu_z|
-           // inside it (glass_entry_cos); 0 otherwise
+  R cmin;  // entering a glass layer denser than the denser medium that
+           // bounds its run of glass layers: the least |mu_z| inside it
+           // (glass_entry_cos, GlassGuard); 0 otherwise
   R mut;   // rescales the remaining step s on transmission: mu_t here /
            // mu_t there when both layers scatter, 0 out of a glass layer, 1
            // into one; -1 where the packet leaves the stack (the interface
@@ -397,10 +406,11 @@ __device__ __forceinline__ R spin_z(R mu_z, R cos_t, R u_phi) {
   return DM<R>::abs(mu_z) > (R)0.99999 ? (mu_z > (R)0 ? cos_t : -cos_t) : general;
 }
 
-// cosine inside a glass slide of index ng entered from index na < ng,
-// clamped away from the critical angle of the way back (see GlassGuard):
-// at least cmin = sqrt(1 - (1 - guard) (na/ng)^2), which the host derives
-// per interface (Iface::cmin, Point::ent_cmin; 0 where no bound applies)
+// cosine inside a glass layer of index ng, clamped away from the critical
+// angle of the denser medium (index nx < ng) that bounds its run of glass
+// layers (see GlassGuard): at least cmin = sqrt(1 - (1 - guard) (nx/ng)^2),
+// which the host derives per interface (Iface::cmin, Point::ent_cmin; 0
+// where no bound applies)
 template <typename R>
 __device__ __forceinline__ R glass_entry_cos(R ct, R cmin) {
   return ct > cmin ? ct : cmin;  // a compare and a select (fmax adds two canonicalizations)
@@ -1147,19 +1157,21 @@ static int validate(ismc_ctx* c, const ismc_config* cfg) {
 }
 
 // Constants of the interface from index na (a layer of extinction ta, or
-// the ambient medium) to index nb (a layer of extinction tb, a slide, or
-// the ambient medium), in FP64 (the same expressions and order as the
+// the ambient medium) to index nb (a layer of extinction tb, a glass layer,
+// or the ambient medium), in FP64 (the same expressions and order as the
 // oracle's, so the FP64 kernel draws its decisions from equal values).
+// n_guard: for a glass layer b, the denser medium bounding its run of glass
+// layers (glass_guard_index).
 template <typename R>
 static Iface<R> make_iface(double na, double nb, bool a_turbid, double ta, bool b_turbid,
-                           bool b_glass, double tb) {
+                           bool b_glass, double tb, double n_guard) {
   Iface<R> f;
   f.eta = (R)(na / nb);
   const double q = (nb - na) / (nb + na);
   f.qq = (R)(q * q);
   double cmin = 0.0;
-  if (b_glass && nb > na) {
-    const double r = na / nb;
+  if (b_glass && nb > n_guard) {
+    const double r = n_guard / nb;
     cmin = std::sqrt(std::max(0.0, 1.0 - (1.0 - (double)GlassGuard<R>::value) * r * r));
   }
   f.cmin = (R)cmin;
@@ -1168,6 +1180,19 @@ static Iface<R> make_iface(double na, double nb, bool a_turbid, double ta, bool 
   // first step in the layer behind is the zero step), 1 into a slide
   f.mut = a_turbid ? (b_turbid ? (R)(ta / tb) : (R)1) : (R)0;
   return f;
+}
+
+// The index that bounds the refracted direction in glass layer g: the
+// denser of the two media (scattering layers, or the ambient medium past
+// the stack's faces) on either side of the run of adjacent glass layers
+// that holds g (GlassGuard).
+static double glass_guard_index(const ismc_config* cfg, int g) {
+  int lo = g, hi = g;
+  while (lo >= 0 && cfg->mu_s[lo] == 0.0) --lo;
+  while (hi < cfg->n_layers && cfg->mu_s[hi] == 0.0) ++hi;
+  const double n_lo = lo >= 0 ? cfg->n[lo] : cfg->n_ambient;
+  const double n_hi = hi < cfg->n_layers ? cfg->n[hi] : cfg->n_ambient;
+  return n_lo > n_hi ? n_lo : n_hi;
 }
 
 template <typename R>
@@ -1217,9 +1242,10 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
     for (int dir = 0; dir < 2; ++dir) {
       const int j = dir ? i + 1 : i - 1;
       const bool inside = j >= 0 && j < L;
+      const bool j_glass = inside && cfg->mu_s[j] == 0.0;
       P->layer[i].out[dir] = make_iface<R>(cfg->n[i], inside ? cfg->n[j] : namb, turbid, cfg->mu_t[i],
-                                           inside && cfg->mu_s[j] != 0.0, inside && cfg->mu_s[j] == 0.0,
-                                           inside ? cfg->mu_t[j] : 1.0);
+                                           inside && cfg->mu_s[j] != 0.0, j_glass, inside ? cfg->mu_t[j] : 1.0,
+                                           j_glass ? glass_guard_index(cfg, j) : 0.0);
       if (!inside) P->layer[i].out[dir].mut = (R)-1;
       else if (turbid && P->layer[j].kind == KIND_SLIDE) P->layer[i].out[dir].mut = (R)INFINITY;
     }
@@ -1240,8 +1266,9 @@ static void compile_point(const ismc_config* cfg, uint32_t flags, Point<R>* P) {
       const double n2 = cfg->n[first[k]];
       const double n3 = (behind[k] >= 0 && behind[k] < L) ? cfg->n[behind[k]] : namb;
       const bool slide = cfg->mu_s[first[k]] == 0.0;
-      const Iface<R> e1 = make_iface<R>(namb, n2, false, 1.0, false, slide, 1.0);
-      const Iface<R> e2 = make_iface<R>(n2, n3, false, 1.0, false, false, 1.0);
+      const Iface<R> e1 = make_iface<R>(namb, n2, false, 1.0, false, slide, 1.0,
+                                        slide ? glass_guard_index(cfg, first[k]) : 0.0);
+      const Iface<R> e2 = make_iface<R>(n2, n3, false, 1.0, false, false, 1.0, 0.0);
       P->ent_eta1[k] = e1.eta; P->ent_qq1[k] = e1.qq; P->ent_cmin[k] = e1.cmin;
       P->ent_eta2[k] = e2.eta; P->ent_qq2[k] = e2.qq;
     }
