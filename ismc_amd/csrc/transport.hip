@@ -353,7 +353,9 @@ __device__ __forceinline__ R entrance_R(const Point<R>& P, int side, R mu_z, R& 
   if (P.ent_glass[side]) {  // the slide's far face only where there is a slide
     R ct2;
     const R r2 = fresnel<R>(P.ent_eta2[side], P.ent_qq2[side], mu_z, ct2);
-    r = r + ((R)1 - r) * ((R)1 - r) * r2 * DM<R>::rcp((R)1 - r * r2);
+    // r = 1 (an ambient medium denser than the slide, at grazing incidence)
+    // reflects totally; with r2 = 1 too the series would read 0/0
+    r = r < (R)1 ? r + ((R)1 - r) * ((R)1 - r) * r2 * DM<R>::rcp((R)1 - r * r2) : (R)1;
   }
   return r;
 }
@@ -893,7 +895,9 @@ __global__ __launch_bounds__(kBlock, (MinWaves<R, kPerPacket>::value)) void tran
       e_inc += 1;
       R ct;
       const R Re = entrance_R<R>(P, side, muz, ct);
-      if (u0 < Re) {
+      // reflected when u <= Re (u in (0, 1]): a totally reflecting entrance
+      // (Re = 1) reflects every packet, as at an interface
+      if (u0 <= Re) {
         phase = PH_SPHERE;  // su1, su2 hold this event's return draw and new |mu_z|
       } else {
         constexpr uint32_t kLayerBytes = (uint32_t)sizeof(Layer<R>);

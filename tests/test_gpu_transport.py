@@ -315,6 +315,37 @@ def test_no_packet_trapped_at_a_face(engine):
     assert abs(sum(r.mean.values()) - 1.0) < 1e-4
 
 
+def test_denser_ambient_over_a_slide(engine):
+    # An ambient medium (n 1.33) denser than the first slide (glass 1.2967,
+    # glass 1.1711 behind it, then a weakly scattering layer; dual spheres).
+    # Where the ambient face reflects totally the entrance reflects every
+    # packet; there the far face reflects totally too (both terms are taken
+    # at the ambient-side cosine) and the slide series once read 0/0: a NaN
+    # let reinjected packets enter along the slide's faces, where they
+    # bounced until the event limit (about 1,600 of 2^16 per launch).  The
+    # kernel's entrance reflectance equals the oracle's (finite, 1 past the
+    # critical angle), and no packet is dropped.
+    from ismc_amd.workloads import standard_sphere
+    cfg = SampleConfig(layers=[Layer.glass(1.2967, 0.005), Layer.glass(1.1711, 0.01),
+                               Layer(1.2029, -0.46, 0.017, 0.02, 0.018)],
+                       R=standard_sphere(), T=standard_sphere(), n_ambient=1.33)
+    mus = np.concatenate([np.linspace(0.01, 1, 60), -np.linspace(0.01, 1, 60)])
+    rows = np.stack([mus, 0 * mus, 0 * mus, 0 * mus], 1)
+    refe = np.array([O.entrance_reflectance(spec_of(cfg), m) for m in mus])
+    assert np.all(np.isfinite(refe)) and np.sum(refe == 1.0) > 20
+    e64 = engine.eval_physics(4, rows, cfg=cfg, fp64=True)[:, 0]
+    e32 = engine.eval_physics(4, rows, cfg=cfg, fp64=False)[:, 0]
+    assert np.all(np.isfinite(e64)) and np.max(np.abs(e64 - refe)) < 1e-12
+    # FP32 1 - sin^2 cancels near each face's critical angle (2.3e-5 there)
+    assert np.all(np.isfinite(e32)) and np.max(np.abs(e32 - refe)) < 1e-4
+    for fp64 in (False, True):
+        for sb in (False, True):
+            r = engine.simulate(cfg, 1 << 16, seed=3, fp64=fp64, slide_bounces=sb)
+            print("fp64", fp64, "slide_bounces", sb, "dropped", r.n_aborted, "sum", sum(r.mean.values()))
+            assert r.n_aborted == 0
+            assert abs(sum(r.mean.values()) - 1.0) < 1e-3
+
+
 def _random_stacks(seed, count):
     # 1-3 layers, each glass (30 %: n 1.40-1.65, 0.01-0.2 cm) or scattering
     # (n 1.30-1.60, g 0-0.95, 0.01-0.2 cm, mu_a 0.01-10, mu_s 1-1000 /cm,

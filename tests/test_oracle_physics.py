@@ -71,6 +71,30 @@ def test_glass_entrance_closed_form():
     assert O.entrance_reflectance(spec1, 1.0) == pytest.approx(r0(1, 1.4), rel=1e-12)
 
 
+def test_glass_entrance_under_a_denser_ambient():
+    # An ambient medium denser than the first slide (n 1.33 over glass
+    # 1.2967, glass 1.1711 behind it): the series holds at normal incidence,
+    # and where the ambient face reflects totally (sin^2 > (1.2967/1.33)^2 at
+    # the ambient-side cosine, both terms taken there) the entrance reflects
+    # totally, although the far face then reflects totally too and the
+    # series reads 0/0
+    spec = {"n_ambient": 1.33,
+            "layers": [{"n": 1.2967, "g": 0, "t": .005, "mu_a": 0, "mu_s": 0},
+                       {"n": 1.1711, "g": 0, "t": .01, "mu_a": 0, "mu_s": 0},
+                       {"n": 1.2029, "g": -.46, "t": .017, "mu_a": .02, "mu_s": .018}]}
+    r1, r2 = r0(1.33, 1.2967), r0(1.2967, 1.1711)
+    assert O.entrance_reflectance(spec, 1.0) == pytest.approx(r1 + (1 - r1) ** 2 * r2 / (1 - r1 * r2), rel=1e-12)
+    mu_c = math.sqrt(1 - (1.2967 / 1.33) ** 2)
+    for mu in (0.01, 0.1, 0.5 * mu_c, 0.99 * mu_c):
+        assert O.entrance_reflectance(spec, mu) == 1.0
+    # past mu_c the far face alone reflects totally up to mu = 0.43 (the
+    # series is then r1 + (1 - r1) = 1); at 0.5 both faces transmit
+    assert O.entrance_reflectance(spec, 1.01 * mu_c) == pytest.approx(1.0, abs=1e-15)
+    r1, r2 = O.fresnel(1.33, 1.2967, 0.5), O.fresnel(1.2967, 1.1711, 0.5)
+    assert r1 < 1 and r2 < 1
+    assert O.entrance_reflectance(spec, 0.5) == pytest.approx(r1 + (1 - r1) ** 2 * r2 / (1 - r1 * r2), rel=1e-12)
+
+
 @pytest.mark.parametrize("g", [-0.5, 0.0, 0.3, 0.8, 0.9, 0.95])
 def test_hg_moments(g):
     # E[cos] = g and E[P2(cos)] = g^2 for Henyey-Greenstein; the sampler is the
