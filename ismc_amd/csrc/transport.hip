@@ -130,9 +130,10 @@ enum : int32_t { KIND_HG = 0, KIND_ISO = 1, KIND_GLASS = 2, KIND_SLIDE = 3 };
 // round apart it drifts the same way at every round trip, until neither
 // face that bounds the run lets the packet out.  The refracted direction in
 // a glass layer is therefore kept at least GlassGuard (relative, in sin^2)
-// inside the critical cone of the denser of the two media that bound its
-// run of glass layers (scattering layers or the ambient medium), through
-// which any packet in the run can then leave.  For a slide entered from a
+// inside the critical cone of the layer's exit index (glass_guard_index):
+// on one side of it every index out to the medium that bounds its run of
+// glass layers (a scattering layer or the ambient medium) is at least that
+// large, so the packet can leave the run there.  For a slide entered from a
 // scattering layer at least as dense as the medium beyond the slide this is
 // the cone of the face the packet came through; it moves only packets
 // within ~sqrt(GlassGuard) of grazing incidence.
@@ -156,8 +157,8 @@ template <typename R>
 struct alignas(16) Iface {
   R eta;   // n_here / n_there (Snell)
   R qq;    // ((n_there - n_here) / (n_there + n_here))^2, normal incidence
-  R cmin;  // entering a glass layer denser than the denser medium that
-           // bounds its run of glass layers: the least |mu_z| inside it
+  R cmin;  // entering a glass layer denser than its exit index
+           // (glass_guard_index): the least |mu_z| inside it
            // (glass_entry_cos, GlassGuard); 0 otherwise
   R mut;   // rescales the remaining step s on transmission: mu_t here /
            // mu_t there when both layers scatter, 0 out of a glass layer, 1
@@ -409,8 +410,8 @@ __device__ __forceinline__ R spin_z(R mu_z, R cos_t, R u_phi) {
 }
 
 // cosine inside a glass layer of index ng, clamped away from the critical
-// angle of the denser medium (index nx < ng) that bounds its run of glass
-// layers (see GlassGuard): at least cmin = sqrt(1 - (1 - guard) (nx/ng)^2),
+// angle of its exit index nx < ng (glass_guard_index, GlassGuard): at least
+// cmin = sqrt(1 - (1 - guard) (nx/ng)^2),
 // which the host derives per interface (Iface::cmin, Point::ent_cmin; 0
 // where no bound applies)
 template <typename R>
@@ -1164,8 +1165,7 @@ static int validate(ismc_ctx* c, const ismc_config* cfg) {
 // the ambient medium) to index nb (a layer of extinction tb, a glass layer,
 // or the ambient medium), in FP64 (the same expressions and order as the
 // oracle's, so the FP64 kernel draws its decisions from equal values).
-// n_guard: for a glass layer b, the denser medium bounding its run of glass
-// layers (glass_guard_index).
+// n_guard: for a glass layer b, its exit index (glass_guard_index).
 template <typename R>
 static Iface<R> make_iface(double na, double nb, bool a_turbid, double ta, bool b_turbid,
                            bool b_glass, double tb, double n_guard) {
@@ -1186,17 +1186,26 @@ static Iface<R> make_iface(double na, double nb, bool a_turbid, double ta, bool 
   return f;
 }
 
-// The index that bounds the refracted direction in glass layer g: the
-// denser of the two media (scattering layers, or the ambient medium past
-// the stack's faces) on either side of the run of adjacent glass layers
-// that holds g (GlassGuard).
+// The index that bounds the refracted direction in glass layer g
+// (GlassGuard): on each side, the least index from g's neighbour out to
+// the medium that bounds g's run of adjacent glass layers (a scattering
+// layer, or the ambient medium past the stack's face), that medium
+// included; the larger of the two.  A packet with n^2 (1 - mu_z^2) below
+// its square crosses every face on that side and leaves the run there.
 static double glass_guard_index(const ismc_config* cfg, int g) {
-  int lo = g, hi = g;
-  while (lo >= 0 && cfg->mu_s[lo] == 0.0) --lo;
-  while (hi < cfg->n_layers && cfg->mu_s[hi] == 0.0) ++hi;
-  const double n_lo = lo >= 0 ? cfg->n[lo] : cfg->n_ambient;
-  const double n_hi = hi < cfg->n_layers ? cfg->n[hi] : cfg->n_ambient;
-  return n_lo > n_hi ? n_lo : n_hi;
+  double side[2];
+  for (int s = 0; s < 2; ++s) {
+    const int step = s ? 1 : -1;
+    double m = INFINITY;
+    int j = g + step;
+    for (; j >= 0 && j < cfg->n_layers; j += step) {
+      m = std::min(m, cfg->n[j]);
+      if (cfg->mu_s[j] != 0.0) break;  // the scattering layer that bounds the run
+    }
+    if (j < 0 || j >= cfg->n_layers) m = std::min(m, cfg->n_ambient);  // the run reaches the stack's face
+    side[s] = m;
+  }
+  return std::max(side[0], side[1]);
 }
 
 template <typename R>

@@ -89,6 +89,8 @@ double oracle_entrance_reflectance(const oracle_config* c, double mu_z)
     return entrance_R_d(c, layer, mu_z);
 }
 
+double oracle_glass_guard_index(const oracle_config* c, int g) { return glass_guard_index_d(c, g); }
+
 double oracle_hg_cos(double g, double u)
 {
     if (g == 0.0) return 2 * u - 1;

@@ -85,6 +85,8 @@ void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t
 double oracle_fresnel(double ni, double nt, double mu_z);
 float oracle_fresnel_f(float ni, float nt, float mu_z);
 double oracle_entrance_reflectance(const oracle_config* c, double mu_z);
+/* the exit index that bounds the direction inside glass layer g (the glass-entry guard) */
+double oracle_glass_guard_index(const oracle_config* c, int g);
 double oracle_hg_cos(double g, double u);
 
 /* per_packet (nullable): n_packets rows of {A, R_d, R_s, T_d, T_u, events} */

@@ -84,6 +84,8 @@ def lib():
         _lib.oracle_fresnel_f.argtypes = [ctypes.c_float] * 3
         _lib.oracle_entrance_reflectance.restype = ctypes.c_double
         _lib.oracle_entrance_reflectance.argtypes = [ctypes.POINTER(_Config), ctypes.c_double]
+        _lib.oracle_glass_guard_index.restype = ctypes.c_double
+        _lib.oracle_glass_guard_index.argtypes = [ctypes.POINTER(_Config), ctypes.c_int]
         _lib.oracle_hg_cos.restype = ctypes.c_double
         _lib.oracle_hg_cos.argtypes = [ctypes.c_double, ctypes.c_double]
     return _lib
@@ -167,6 +169,11 @@ def fresnel(ni, nt, mu_z, fp32=False):
 def entrance_reflectance(spec, mu_z):
     c = make_config(spec)
     return lib().oracle_entrance_reflectance(ctypes.byref(c), mu_z)
+
+
+def glass_guard_index(spec, layer):
+    c = make_config(spec)
+    return lib().oracle_glass_guard_index(ctypes.byref(c), int(layer))
 
 
 def hg_cos(g, u):

@@ -71,6 +71,30 @@ def test_glass_entrance_closed_form():
     assert O.entrance_reflectance(spec1, 1.0) == pytest.approx(r0(1, 1.4), rel=1e-12)
 
 
+def test_glass_guard_exit_index():
+    # The glass-entry guard keeps n^2 (1 - mu_z^2) inside a glass layer below
+    # the square of its exit index: on each side the least index from the
+    # neighbour out to the medium that bounds the layer's run of glass
+    # layers (that medium included), the larger of the two sides.  A packet
+    # below it crosses every face on that side.
+    def T(n):
+        return {"n": n, "g": .5, "t": .01, "mu_a": .1, "mu_s": 10}
+
+    def G(n):
+        return {"n": n, "g": 0, "t": .01, "mu_a": 0, "mu_s": 0}
+
+    assert O.glass_guard_index({"layers": [G(1.52), T(1.40)]}, 0) == 1.40          # air | slide | sample
+    assert O.glass_guard_index({"layers": [T(1.33), G(1.52), T(1.40)]}, 1) == 1.40  # between two samples
+    s = {"layers": [G(1.4675), G(1.4693), T(1.433)]}                               # two slides on a sample
+    assert [O.glass_guard_index(s, k) for k in (0, 1)] == [1.433, 1.433]
+    # a run with a low-index layer inside: from 1.6318 and 1.4731 the way
+    # out to the left passes 1.1919, so the bound is the right side's 1.1968
+    s = {"layers": [T(1.3755), G(1.444), G(1.1919), G(1.6318), G(1.4731), T(1.1968)]}
+    assert [O.glass_guard_index(s, k) for k in (1, 2, 3, 4)] == [1.3755, 1.3755, 1.1968, 1.1968]
+    s = {"n_ambient": 1.33, "layers": [G(1.2967), G(1.1711), T(1.2029)]}             # denser ambient
+    assert [O.glass_guard_index(s, k) for k in (0, 1)] == [1.33, 1.2967]
+
+
 def test_glass_entrance_under_a_denser_ambient():
     # An ambient medium denser than the first slide (n 1.33 over glass
     # 1.2967, glass 1.1711 behind it): the series holds at normal incidence,
