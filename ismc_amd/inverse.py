@@ -355,23 +355,53 @@ def refine_mc_newton(problems, start, iters=8, packets=100_000, seed=7, engine: 
 
 
 def check(problems, sol, packets=1_000_000, seed=9, engine: Optional[Engine] = None,
-          refine_packets=100_000, z=4.0, floor=0.002):
+          refine_packets=100_000, z=4.0, floor=0.002, screen_z=1.5):
     """Independent MC at each solution.  A solution misses when R_d or T_d
     differs from the measurement by more than z combined MC standard errors
     (this check's and the refinement's, which scales as sqrt(packets /
-    refine_packets)) plus a relative floor.  Returns (miss, relative
-    misfits R, T, packets)."""
+    refine_packets)) plus a relative floor.
+
+    Two stages: every solution is first re-simulated at refine_packets on
+    streams of its own, and passes when both readings lie within screen_z
+    combined standard errors (that run's and the refinement's) plus the
+    floor; only the others are judged at `packets` as above (on the same
+    streams and seed as a one-stage check, so their verdicts are the ones
+    it gives).  A solution that the full check would fail is off by more
+    than z (1 + refine_packets / packets)^(1/2) refinement standard errors
+    (4.2 for z = 4 and packets = 10 refine_packets) beyond the floor; it
+    passes the screen only where the screen's reading falls short of that
+    by more than 4.2 - screen_z 2^(1/2) = 2.08 of its standard errors,
+    odds of 1.9 % at the threshold and less beyond it.  A solution that
+    fits (residual of about one refinement standard error) passes with
+    odds of about 0.87 per reading, so about three in four skip the full
+    check.  screen_z=None checks every solution at `packets`.  Returns
+    (miss, relative misfits R, T, packets), the misfits from the last
+    stage each solution ran."""
     eng = engine or default_engine()
     sol = np.asarray(sol, dtype=np.float64)
     P = len(problems)
-    F, SE = mc_rt(eng, problems, sol[:, 0], sol[:, 1], packets, seed,
-                  np.arange(P, dtype=np.uint32) + 90_000_000, use_graph=False)
     Mv = np.array([[p.measured.R_d, p.measured.T_d] for p in problems], dtype=np.float64)
-    k = math.sqrt(1.0 + packets / float(refine_packets))
-    d = np.abs(F - Mv)
-    rel = d / np.maximum(Mv, 1e-12)
-    miss = np.any(d > z * k * SE + floor * Mv, axis=1)
-    return miss, rel[:, 0], rel[:, 1], P * packets
+    rel = np.zeros((P, 2))
+    todo = np.arange(P)
+    used = 0
+    if screen_z is not None and refine_packets < packets and P:
+        F1, SE1 = mc_rt(eng, problems, sol[:, 0], sol[:, 1], refine_packets, seed + 1,
+                        np.arange(P, dtype=np.uint32) + 95_000_000, use_graph=False)
+        used += P * refine_packets
+        d1 = np.abs(F1 - Mv)
+        rel = d1 / np.maximum(Mv, 1e-12)
+        clear = np.all(d1 <= screen_z * math.sqrt(2.0) * SE1 + floor * Mv, axis=1)
+        todo = np.where(~clear)[0]
+    miss = np.zeros(P, dtype=bool)
+    if len(todo):
+        F, SE = mc_rt(eng, problems, sol[todo, 0], sol[todo, 1], packets, seed,
+                      todo.astype(np.uint32) + 90_000_000, ids=todo, use_graph=False)
+        used += len(todo) * packets
+        k = math.sqrt(1.0 + packets / float(refine_packets))
+        d = np.abs(F - Mv[todo])
+        rel[todo] = d / np.maximum(Mv[todo], 1e-12)
+        miss[todo] = np.any(d > z * k * SE + floor * Mv[todo], axis=1)
+    return miss, rel[:, 0], rel[:, 1], used
 
 
 GENERIC_GUESS = (0.3, 5.0)
