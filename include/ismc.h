@@ -92,7 +92,11 @@ typedef struct {
     uint64_t events[5];   /* interactions, boundary hits, entrances,
                              sphere decisions, re-injections            */
     uint64_t n_packets;
-    uint64_t n_aborted;   /* packets stopped by the per-packet event cap */
+    uint64_t n_aborted;   /* packets stopped by the per-packet event cap
+                             (2^22 events; not tallied).  Reached only by
+                             genuinely long histories, e.g. a thin, weakly
+                             scattering layer denser than both neighbours,
+                             which guides packets by total reflection */
 } ismc_tally;
 
 /* Launch shape; zero fields mean "use the context's default".  Tallies do
