@@ -223,7 +223,7 @@ def refine_batch(problems, seeds, packets=100_000, step=0.03, tol=3e-3, seed=7,
 
 
 def refine_newton(problems, seeds, sur, iters=6, packets=100_000, seed=7, engine: Optional[Engine] = None,
-                  stream_base=0, g=G, n=N_INDEX, t=THICK, radius=0.5, s_cut=0.05, log=None):
+                  stream_base=0, g=G, n=N_INDEX, t=THICK, radius=0.5, s_cut=0.05, log=None, step_tol=0.0):
     """Defect correction with the surrogate's Jacobian, in y = (ln mu_a, ln mu_s):
     r = (MC(y) - M) / M are the relative residuals of R_d, T_d (every solve's
     MC in one launch per iteration, common random numbers per solve) and J
@@ -233,7 +233,9 @@ def refine_newton(problems, seeds, sur, iters=6, packets=100_000, seed=7, engine
     absorption hardly changes R_d and T_d, is left alone), limited to a
     trust radius per solve.  A trial whose residual norm is not smaller is
     rejected and the radius shrinks fourfold; an accepted one doubles it.
-    Returns (best solutions, evaluations per solve, stats)."""
+    A solve whose next step is shorter than step_tol (in ln mu) stops
+    there and runs in no later launch (0: every solve runs every
+    iteration).  Returns (best solutions, evaluations per solve, stats)."""
     eng = engine or default_engine()
     P = len(problems)
     Mv = np.array([[p.measured.R_d, p.measured.T_d] for p in problems])
@@ -248,19 +250,28 @@ def refine_newton(problems, seeds, sur, iters=6, packets=100_000, seed=7, engine
     h = 0.05
     lo, hi = np.array([[-9.2, -6.9]]), np.array([[6.9, 9.2]])
     cache = {}
+    active = np.ones(P, dtype=bool)
+    nev = np.zeros(P, dtype=np.int64)
     for it in range(iters):
-        mua, mus = np.exp(y_try[:, 0]), np.exp(y_try[:, 1])
-        F, _ = mc_rt(eng, problems, mua, mus * (1 - g), packets, seed, streams, cache=cache)
+        ids = np.where(active)[0]
+        if len(ids) == 0:
+            break
+        mua, mus = np.exp(y_try[ids, 0]), np.exp(y_try[ids, 1])
+        Fi, _ = mc_rt(eng, problems, mua, mus * (1 - g), packets, seed, streams[ids],
+                      ids=None if len(ids) == P else ids, cache=cache)
+        F = np.full((P, 2), np.inf)
+        F[ids] = Fi
+        nev[ids] += 1
         stats["launches"] += 1
-        stats["packets"] += packets * P
+        stats["packets"] += packets * len(ids)
         stats["kernel_ms"] += _last_ms(eng)
         r = (F - Mv) / Mv
-        better = np.linalg.norm(r, axis=1) < np.linalg.norm(r_acc, axis=1)
+        better = active & (np.linalg.norm(r, axis=1) < np.linalg.norm(r_acc, axis=1))
         if it > 0:
-            stats["rejected"] += int((~better).sum())
+            stats["rejected"] += int((active & ~better).sum())
         y_acc[better] = y_try[better]
         r_acc[better] = r[better]
-        rad = np.where(better, np.minimum(rad * 2.0, 1.0), rad * 0.25)
+        rad = np.where(better, np.minimum(rad * 2.0, 1.0), np.where(active, rad * 0.25, rad))
         if it == iters - 1:
             break
         # surrogate Jacobian d(R, T)/dy at the accepted points, all solves at once
@@ -273,12 +284,15 @@ def refine_newton(problems, seeds, sur, iters=6, packets=100_000, seed=7, engine
         dy = -np.einsum("pji,pj,pjk,pk->pi", Wt, Sinv, U, r_acc)
         nrm = np.linalg.norm(dy, axis=1)
         dy *= np.minimum(1.0, rad / np.maximum(nrm, 1e-300))[:, None]
-        y_try = np.clip(y_acc + dy, lo, hi)
+        if step_tol > 0:
+            active &= np.linalg.norm(dy, axis=1) >= step_tol
+        y_try = np.where(active[:, None], np.clip(y_acc + dy, lo, hi), y_acc)
         if log:
-            log("    defect correction %d: median |r| %.2e, median |dy| %.2e, %d rejected" %
-                (it, float(np.median(np.abs(r_acc).max(1))), float(np.median(nrm)), int((~better).sum())))
+            log("    defect correction %d: median |r| %.2e, median |dy| %.2e, %d rejected, %d running" %
+                (it, float(np.median(np.abs(r_acc).max(1))), float(np.median(nrm)), int((active & ~better).sum()),
+                 int(active.sum())))
     sol = np.stack([np.exp(y_acc[:, 0]), np.exp(y_acc[:, 1]) * (1 - g)], 1)
-    return sol, np.full(P, iters), stats
+    return sol, nev, stats
 
 
 def refine_mc_newton(problems, start, iters=8, packets=100_000, seed=7, engine: Optional[Engine] = None,
@@ -529,12 +543,22 @@ def summary(ea, es):
             "frac_both_within_2pct": float(np.mean((ea <= 0.02) & (es <= 0.02)))}
 
 
+# Defect correction stops a solve once its next step is shorter than this
+# in ln mu (0.1 %): a tenth of the accuracy the solves are judged at, and
+# below what the refinement's MC evaluations resolve.  Measured on 4096
+# solves against refining every solve six times: 4.1 evaluations per solve
+# instead of 6 at 1 % noise (solves within 1 % of both parameters 339
+# against 344, median errors and fit unchanged) and 2.7 at 0.1 % noise
+# (1647 against 1635).
+REFINE_STEP_TOL = 1e-3
+
+
 def _solve_seeded(probs, sur, seeds, method, refine_packets, check_packets, eng, log, stream_base,
                   fallback="newton"):
     t0 = time.perf_counter()
     if method == "newton":
         sol, nev, st = refine_newton(probs, seeds, sur, packets=refine_packets, engine=eng,
-                                     stream_base=stream_base, log=log)
+                                     stream_base=stream_base, log=log, step_tol=REFINE_STEP_TOL)
     else:
         sol, nev, st = refine_batch(probs, seeds, packets=refine_packets, engine=eng, log=log,
                                     stream_base=stream_base)

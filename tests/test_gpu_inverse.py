@@ -104,8 +104,9 @@ def test_grid_argmin_equals_numpy(engine):
 
 def test_full_size_pipeline(engine):
     # The inverse workload at its full size: 4096 synthetic measurements
-    # (1e6 packets, 1 % noise) through the surrogate's seeds, six batched MC
-    # defect-correction launches, the MC judge at 1e6 packets and the
+    # (1e6 packets, 1 % noise) through the surrogate's seeds, up to six
+    # batched MC defect-correction launches (a solve whose step falls below
+    # 0.1 % stops earlier), the two-stage MC judge and the
     # fallback on its misses.  Checked: the solutions fit the measurements
     # as well as the truth does in most cases, the recovered scattering is
     # accurate and the judge's misses are few and mostly repaired.
@@ -115,7 +116,7 @@ def test_full_size_pipeline(engine):
     s = r["seeded_newton"]
     print({k: s[k] for k in ("solves_per_s", "misses", "misses_repaired_by_fallback", "fits_as_well_as_truth")},
           s["errors"])
-    assert s["mean_evals"] == 6 and s["launches"] == 6
+    assert s["launches"] <= 6 and 2 < s["mean_evals"] < 6
     assert s["fits_as_well_as_truth"] > 0.78
     assert s["errors"]["median_mus_reduced"] < 0.025 and s["errors"]["median_mu_a"] < 0.12
     assert s["misses"] < 100 and s["misses_repaired_by_fallback"] >= s["misses"] // 3
