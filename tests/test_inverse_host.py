@@ -200,3 +200,44 @@ def test_wilson_and_ratio_intervals():
     r = (339 / 4096) / (35 / 512)
     # Katz: exp(+-1.96 sqrt(1/339 - 1/4096 + 1/35 - 1/512)) = 0.7149, 1.3988
     assert abs(lo / r - 0.7149) < 1e-3 and abs(hi / r - 1.3988) < 1e-3
+
+
+def _analytic_problems(eng, truths):
+    probs = []
+    for a, s in truths:
+        t = eng.tallies(inverse.problem(0.5, 0.5).configs(a, s)[0][0])
+        probs.append(inverse.problem(t["R_d"], t["T_d"]))
+    return probs
+
+
+def test_check_screens_clear_fits_and_judges_the_rest():
+    # exact solutions clear the screen (one launch of every solution at the
+    # refinement's packets); solutions 20 % off in mu_s' do not, go to the
+    # full check (a second launch of just those) and miss; the one-stage
+    # check gives the same verdicts in one launch of every solution
+    eng = AnalyticEngine()
+    truths = np.array([(0.05, 2.0), (0.5, 20.0), (3.0, 80.0), (0.2, 150.0), (1.0, 10.0)])
+    probs = _analytic_problems(eng, truths)
+    sol = truths * np.array([[1, 1], [1, 1.2], [1, 1], [1, 0.8], [1, 1]])
+    miss, rR, rT, used = inverse.check(probs, sol, 1_000_000, engine=eng, refine_packets=100_000)
+    assert eng.calls == [5, 2]
+    assert miss.tolist() == [False, True, False, True, False]
+    assert used == 5 * 100_000 + 2 * 1_000_000
+    assert rR[[0, 2, 4]].max() < 1e-12 and rR[1] > 0.01
+    eng.calls.clear()
+    miss1, _, _, used1 = inverse.check(probs, sol, 1_000_000, engine=eng, refine_packets=100_000, screen_z=None)
+    assert eng.calls == [5] and miss1.tolist() == miss.tolist() and used1 == 5 * 1_000_000
+
+
+def test_defect_correction_stops_below_the_step_tolerance():
+    # with an exact model the steps shrink quadratically: once a solve's
+    # next step is below step_tol it runs in no later launch, and it stops
+    # within about that step of the truth
+    eng = AnalyticEngine()
+    truths = np.array([(0.05, 2.0), (0.5, 20.0), (3.0, 80.0)])
+    probs = _analytic_problems(eng, truths)
+    sol, nev, st = inverse.refine_newton(probs, truths * [1.2, 0.85], AnalyticSurrogate(), iters=8,
+                                         packets=1000, engine=eng, s_cut=1e-6, step_tol=1e-3)
+    assert st["launches"] < 8 and nev.max() < 8 and nev.min() >= 2
+    assert eng.calls == sorted(eng.calls, reverse=True) and eng.calls[0] == 3
+    assert np.allclose(sol, truths, rtol=3e-3)
